@@ -1,0 +1,185 @@
+/* C ABI of libphaser_mi355x.so: HIP (gfx950) kernels for phaser's reconstruction hot path.
+ *
+ * Every entry point takes device pointers, plain sizes and a hipStream_t, launches its
+ * kernel(s) on that stream and returns 0 on success, PHX_EINVAL (-1000) for a bad
+ * argument, PHX_EUNSUPPORTED (-1001) for a size without a kernel, or a negative
+ * hipError_t.  Nothing allocates, nothing synchronises, no C++ exception crosses the ABI.
+ * Complex arrays are interleaved complex64 (float re, float im), C-contiguous.
+ *
+ * Reference citations are to wdwzyyg/phaser.
+ */
+#ifndef PHASER_MI355X_H
+#define PHASER_MI355X_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------
+ * Object cutouts.  Replace the cupy kernels get_cutouts / add_cutouts / set_cutouts
+ * (phaser/utils/_cuda_kernels.py:19-107, kernel :163-211) behind ObjectCutout.get/add/set
+ * (phaser/utils/object.py:501-557).  start_idxs: (n_cutouts, 2) int64 (y, x) on device;
+ * cutouts: (n_cutouts, n_objects, cut_h, cut_w); obj: (n_objects, obj_h, obj_w).
+ * Out-of-bounds reads give the default value; out-of-bounds adds/sets are skipped.
+ * add_* uses hardware float atomics (summation order is unspecified).
+ * --------------------------------------------------------------------------------- */
+int phx_get_cutouts_f32(const void* obj, void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        float default_value, hipStream_t stream);
+int phx_get_cutouts_c64(const void* obj, void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        float default_re, float default_im, hipStream_t stream);
+int phx_add_cutouts_f32(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream);
+int phx_add_cutouts_c64(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream);
+int phx_set_cutouts_f32(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream);
+int phx_set_cutouts_c64(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream);
+
+/* Fill `count` 32-bit words with `value` (a kernel, never a memset node: safe in graphs). */
+int phx_fill_u32(void* ptr, uint32_t value, int64_t count, hipStream_t stream);
+
+/* Batched fft2 (inverse=0) / ifft2 (inverse=1) with phaser's convention
+ * (phaser/utils/num.py:434-468: centred real space, corner-origin reciprocal space,
+ * norm='ortho') over `batch` n x n planes, n in {16, 32, 64, 128}.  `twiddle` holds the
+ * n complex64 values exp(-2 pi i j / n).  in != out. */
+int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int inverse, const void* twiddle,
+                 hipStream_t stream);
+
+/* Copy n x n planes between natural and scrambled frequency order (see below):
+ * unscramble == 0: out[b][py][px] = in[b][f(py)][f(px)]; unscramble != 0: the inverse.
+ * is_complex selects complex64 (else float32) elements.  in != out, batch <= 65535. */
+int phx_scramble(const void* in, void* out, int64_t batch, int64_t n, int is_complex, int unscramble,
+                 hipStream_t stream);
+
+/* ptr[i] *= scale for `count` complex64 values (presolve probe rescale, solvers.py:60). */
+int phx_scale_c64(void* ptr, int64_t count, float scale, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * Group step of the conventional engines (LSQ-ML: phaser/engines/conventional/
+ * solvers.py:131-290; ePIE: :386-504) and the forward/adjoint model of the gradient
+ * engine (phaser/engines/gradient/run.py:352-438), split into fused plane kernels.
+ *
+ * Frequency-domain arrays (propagators, patterns, mask, k-grids, probe spectra) are kept
+ * in the FFT kernels' "scrambled" order: along each axis of an n-point transform,
+ * position p holds frequency (p / R2) + R1 * (p % R2) where n = R1 * R2 (R1 = 16, 8, 8, 4;
+ * R2 = 8, 8, 4, 4 for n = 128, 64, 32, 16).  Real-space arrays are natural order.
+ * --------------------------------------------------------------------------------- */
+typedef struct phx_group_desc {
+  int32_t n;          /* plane side, 16..128, power of two */
+  int32_t n_modes;    /* probe modes M */
+  int32_t n_slices;   /* object slices S */
+  int32_t obj_h;      /* object slice height H */
+  int32_t obj_w;      /* object slice width W */
+  int32_t bmax;       /* group slots launched (>= every group's size) */
+  int32_t group_index;/* group number, recorded in finite_flag on NaN/inf */
+  int32_t n_pos;      /* total scan positions P */
+  const int32_t* group_pos;  /* (bmax) position index of each slot of this group */
+  const int32_t* group_size; /* device scalar: slots in use */
+  const int32_t* start;      /* (P, 2) cutout start (y, x) */
+  const float* shift;        /* (P, 2) subpixel shift (y, x), length units */
+  const float* ky_s;         /* (n) ky at scrambled row position */
+  const float* kx_s;         /* (n) kx at scrambled column position */
+  const void* twiddle;       /* (n) complex64 exp(-2 pi i j / n) */
+  void* object;              /* (S, H, W) complex64 */
+  void* probe;               /* (M, n, n) complex64 */
+  const void* props;         /* (S-1, n, n) complex64, scrambled; may be NULL when S == 1 */
+  const float* patterns_s;   /* (P, n*n) float32, scrambled */
+  const float* mask_s;       /* (n*n) float32, scrambled */
+  const float* params;       /* device: [beta_obj, beta_probe, reg_obj, reg_probe, gamma,
+                                          noise_offset, noise_eps, pos_scale] */
+  void* psi;                 /* (S, bmax, M, n, n) complex64: stored forward waves */
+  void* chi_a;               /* (bmax, M, n, n) complex64 */
+  void* chi_b;               /* (bmax, M, n, n) complex64 */
+  void* farfield;            /* (bmax, M, n*n) complex64, scrambled exit-wave spectra */
+  float* intensity;          /* (bmax, n*n) float32 model intensity, scrambled */
+  float* gpm;                /* (S, H, W) float32 group illumination */
+  void* d_obj;               /* (H, W) complex64 object update accumulator (kept zero) */
+  void* d_probe;             /* (M, n*n) complex64 probe update spectrum (kept zero) */
+  void* probe_k;             /* (M, n*n) complex64 probe spectra, scrambled */
+  float* gom;                /* (n*n) float32 group object magnitude */
+  double* red;               /* reductions, phx_red_doubles(S, bmax, M) doubles */
+  float* errors;             /* (bmax) per-slot detector error of this group */
+  float* aux;                /* (2*bmax) float32 scratch (ePIE maxima) */
+  int32_t* finite_flag;      /* device int: min group index with NaN/inf (INT32_MAX = none) */
+} phx_group_desc;
+
+/* Doubles needed by phx_group_desc.red. */
+int64_t phx_red_doubles(int32_t n_slices, int32_t bmax, int32_t n_modes);
+
+/* Zero the per-group accumulators (gpm, intensity, red, errors). */
+int phx_group_begin(const phx_group_desc* d, hipStream_t stream);
+
+/* probe_k[m] = FFT(probe[m]) (scrambled, unnormalised).  cutout_group's fft2(probes)
+ * (phaser/engines/common/simulation.py:255). */
+int phx_probe_spectra(const phx_group_desc* d, hipStream_t stream);
+
+/* gom = sum_b |prod_s O_s(cutout b)|^2 (solvers.py:208). */
+int phx_group_obj_mag(const phx_group_desc* d, hipStream_t stream);
+
+/* psi[0] = ifft2(fft2(probe) * shift_filter(b)) (simulation.py:241-260, optics.py:151-170);
+ * optional illumination sum_m |psi0|^2 scattered into gpm[0] (solvers.py:213-215);
+ * norm[b] = sum |psi0|^2; and, when propagate != 0, the slice-0 step
+ * psi[1] = ifft2(fft2(psi0 * O_0) * prop_0) (solvers.py:217-220). */
+int phx_shift_probes(const phx_group_desc* d, int illum, int propagate, hipStream_t stream);
+
+/* Slice step i (1 <= i <= S-2): psi[i+1] = ifft2(fft2(psi[i] * O_i) * prop_i), with the
+ * illumination of slice i when illum != 0 (solvers.py:210-222). */
+int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t stream);
+
+/* Exit wave (solvers.py:231-240): farfield = FFT(psi[S-1] * O_{S-1}), intensity = sum_m
+ * |fft2|^2; illumination of slice S-1 when illum != 0.  Then the amplitude/Anscombe
+ * modulus constraint (noise_models.py:47-63) chi_a = ifft2(update * Psi) and, when
+ * calc_error != 0, errors[b] = sqrt(nansum((I - D)^2)) (solvers.py:237, FP64 sum). */
+int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t stream);
+
+/* LSQ-ML backward step at slice i (solvers.py:242-271).  Reads chi from chi_a (even
+ * steps from the exit) or chi_b; when update_object, accumulates d_obj and the FP64
+ * alpha_O sums; for i > 0 writes the back-propagated chi to the other buffer; for i == 0
+ * and update_probe, accumulates d_probe and the FP64 alpha_P sums. */
+int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
+                       hipStream_t stream);
+/* O_i += beta_O (sum_b alpha_O,b) d_obj/(gpm_i + reg) * gpm_i/(gpm_i + 1e-16); d_obj = 0. */
+int phx_lsqml_object_apply(const phx_group_desc* d, int slice, hipStream_t stream);
+/* P += beta_P (sum_b alpha_P) ifft2(d_probe)/(gom + reg) * gom/(gom + 1e-16); d_probe = 0. */
+int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream);
+
+/* ePIE backward step at slice i (solvers.py:459-485), maxima first, then the update. */
+int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
+                      hipStream_t stream);
+/* O_i += d_obj; d_obj = 0. */
+int phx_epie_object_apply(const phx_group_desc* d, int slice, hipStream_t stream);
+/* P += beta_P ifft2(mean_b d_probe); d_probe = 0. */
+int phx_epie_probe_apply(const phx_group_desc* d, hipStream_t stream);
+
+/* Dry run rescale factors (solvers.py:131-167): rescale[b] = sum(D*mask)/sum|psi[S-1]|^2
+ * into errors[b] (by Parseval sum|fft2(psi)|^2 = sum|psi|^2). */
+int phx_dry_run_stats(const phx_group_desc* d, hipStream_t stream);
+
+/* Position gradient numerators/denominators (solvers.py:275-288) for direction dir
+ * (0 = y, 1 = x), using the final chi (in chi_a when S is odd, chi_b when even). */
+int phx_position_gradient(const phx_group_desc* d, int dir, hipStream_t stream);
+
+/* Model intensity sum_m |fft2(psi[S-1] * O_{S-1})|^2 of each slot, written in natural
+ * (corner-origin) order to out[group_pos[b]] ((P, n*n) float32): the forward model used to
+ * simulate diffraction data (no reference counterpart; the reference's gradient engine
+ * evaluates the same model inside its loss, gradient/run.py:385-393). */
+int phx_exit_intensity(const phx_group_desc* d, float* out, hipStream_t stream);
+
+/* pos_update[(group_pos[b], dir)] = num[b][dir] / den[dir] for the group's slots, after
+ * both phx_position_gradient calls; pos_update is (P, 2) float32. */
+int phx_position_finish(const phx_group_desc* d, void* pos_update, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+// Object cutout gather / scatter-add / set, fills, and the standalone convention-exact
+// batched fft2/ifft2.
+//
+// The cutout entry points replace phaser's cupy kernels get/add/set_cutouts
+// (phaser/utils/_cuda_kernels.py:19-107, kernel template :163-211): same index
+// semantics (start_idxs (n_cutouts, 2) int64 rows of (y, x); cutout c of object slice o
+// reads obj[o, start_y + i, start_x + j]; out-of-bounds reads give `default`, out-of-
+// bounds adds/sets are skipped), same layout (cutouts (n_cutouts, n_objects, cy, cx)).
+#include "phx_common.h"
+#include "phx_fft.h"
+#include "../../include/phaser_mi355x.h"
+
+using namespace phx;
+
+namespace {
+
+template <class Tv>
+struct Ops;
+template <>
+struct Ops<float> {
+  static __device__ __forceinline__ void add(float* p, float v) { atomic_add_f32(p, v); }
+};
+template <>
+struct Ops<float2> {
+  static __device__ __forceinline__ void add(float2* p, float2 v) { atomic_add_c64(p, v); }
+};
+
+enum class CutOp { Get, Add, Set };
+
+// grid: (ceil(cx/64), ceil(cy/4), n_cutouts*n_objects); block (64, 4)
+template <class Tv, CutOp OP>
+__global__ void __launch_bounds__(256) cutout_k(Tv* __restrict__ obj, Tv* __restrict__ cut,
+                                                const int64_t* __restrict__ start, int64_t n_cut,
+                                                int64_t n_obj, int64_t oh, int64_t ow, int64_t ch,
+                                                int64_t cw, Tv dflt) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+  const int64_t cn = blockIdx.z;
+  const int64_t o = cn % n_obj, c = cn / n_obj;
+  if (c >= n_cut || i >= ch || j >= cw) return;
+  const int64_t cidx = (cn * ch + i) * cw + j;
+  const int64_t yi = i + start[2 * c], xj = j + start[2 * c + 1];
+  if (yi < 0 || yi >= oh || xj < 0 || xj >= ow) {
+    if (OP == CutOp::Get) cut[cidx] = dflt;
+    return;
+  }
+  const int64_t oidx = (o * oh + yi) * ow + xj;
+  if (OP == CutOp::Get) cut[cidx] = obj[oidx];
+  else if (OP == CutOp::Add) Ops<Tv>::add(&obj[oidx], cut[cidx]);
+  else obj[oidx] = cut[cidx];
+}
+
+template <class Tv, CutOp OP>
+int launch_cutout(void* obj, void* cut, const int64_t* start, int64_t n_cut, int64_t n_obj, int64_t oh,
+                  int64_t ow, int64_t ch, int64_t cw, Tv dflt, hipStream_t stream) {
+  if (!obj || !cut || !start || n_cut < 0 || n_obj < 1 || oh < 1 || ow < 1 || ch < 1 || cw < 1) return PHX_EINVAL;
+  if (n_cut == 0) return PHX_OK;
+  if (n_cut * n_obj > 0x7fffffffLL || ch > 0x7fffffffLL * 4) return PHX_EINVAL;
+  dim3 block(64, 4, 1);
+  dim3 grid((unsigned)((cw + 63) / 64), (unsigned)((ch + 3) / 4), (unsigned)(n_cut * n_obj));
+  hipLaunchKernelGGL((cutout_k<Tv, OP>), grid, block, 0, stream, (Tv*)obj, (Tv*)cut, start, n_cut, n_obj, oh,
+                     ow, ch, cw, dflt);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+__global__ void fill_u32_k(uint32_t* p, uint32_t v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ----- convention-exact fft2 / ifft2 (phaser/utils/num.py:434-468) for testing -----
+template <int N, int T>
+__global__ void __launch_bounds__(T) fft2_conv_k(const float2* __restrict__ in, float2* __restrict__ out,
+                                                 const float2* __restrict__ twg, int inverse) {
+  __shared__ float2 lds[Plane<N, T>::LDS_FLOAT2 + N];
+  Plane<N, T> pl{lds, lds + Plane<N, T>::LDS_FLOAT2};
+  load_twiddles<N, T>(lds + Plane<N, T>::LDS_FLOAT2, twg);
+  __syncthreads();
+  const float2* a = in + (size_t)blockIdx.x * N * N;
+  float2* b = out + (size_t)blockIdx.x * N * N;
+  const float sc = 1.0f / (float)N;
+  if (!inverse) {
+    // fft2(a) = (-1)^(ky+kx) * FFT(a) / N  for even N
+    pl.forward([&](int y, int x) { return a[y * N + x]; },
+               [&](int py, int px, float2& v) {
+                 const int ky = scr_to_freq<N>(py), kx = scr_to_freq<N>(px);
+                 const float sg = ((ky + kx) & 1) ? -sc : sc;
+                 b[ky * N + kx] = cscale(v, sg);
+               });
+  } else {
+    // ifft2(A) = IFFT((-1)^(ky+kx) * A) / N
+    pl.inverse(
+        [&](int py, int px) {
+          const int ky = scr_to_freq<N>(py), kx = scr_to_freq<N>(px);
+          const float sg = ((ky + kx) & 1) ? -sc : sc;
+          return cscale(a[ky * N + kx], sg);
+        },
+        [&](int y, int x, float2 v) { b[y * N + x] = v; });
+  }
+}
+
+// out[b][py][px] = in[b][freq(py)][freq(px)] (scramble) or the inverse map.
+template <int N, class Tv>
+__global__ void scramble_k(const Tv* __restrict__ in, Tv* __restrict__ out, int unscramble) {
+  const int b = blockIdx.y;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= N * N) return;
+  const int py = e / N, px = e % N;
+  const int fe = scr_to_freq<N>(py) * N + scr_to_freq<N>(px);
+  const size_t base = (size_t)b * N * N;
+  if (unscramble) out[base + fe] = in[base + e];
+  else out[base + e] = in[base + fe];
+}
+
+template <int N>
+int launch_scramble(const void* in, void* out, int64_t batch, int is_complex, int unscramble, hipStream_t stream) {
+  dim3 grid((unsigned)((N * N + 255) / 256), (unsigned)batch);
+  if (is_complex)
+    hipLaunchKernelGGL((scramble_k<N, float2>), grid, dim3(256), 0, stream, (const float2*)in, (float2*)out, unscramble);
+  else
+    hipLaunchKernelGGL((scramble_k<N, float>), grid, dim3(256), 0, stream, (const float*)in, (float*)out, unscramble);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+__global__ void scale_c64_k(float2* p, int64_t n, float s) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = cscale(p[i], s);
+}
+
+template <int N, int T>
+int launch_fft2(const void* in, void* out, int64_t batch, int inverse, const void* tw, hipStream_t stream) {
+  hipLaunchKernelGGL((fft2_conv_k<N, T>), dim3((unsigned)batch), dim3(T), 0, stream, (const float2*)in,
+                     (float2*)out, (const float2*)tw, inverse);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phx_get_cutouts_f32(const void* obj, void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        float default_value, hipStream_t stream) {
+  return launch_cutout<float, CutOp::Get>((void*)obj, cutouts, start_idxs, n_cutouts, n_objects, obj_h, obj_w,
+                                          cut_h, cut_w, default_value, stream);
+}
+
+int phx_get_cutouts_c64(const void* obj, void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        float default_re, float default_im, hipStream_t stream) {
+  return launch_cutout<float2, CutOp::Get>((void*)obj, cutouts, start_idxs, n_cutouts, n_objects, obj_h, obj_w,
+                                           cut_h, cut_w, make_float2(default_re, default_im), stream);
+}
+
+int phx_add_cutouts_f32(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream) {
+  return launch_cutout<float, CutOp::Add>(obj, (void*)cutouts, start_idxs, n_cutouts, n_objects, obj_h, obj_w,
+                                          cut_h, cut_w, 0.f, stream);
+}
+
+int phx_add_cutouts_c64(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream) {
+  return launch_cutout<float2, CutOp::Add>(obj, (void*)cutouts, start_idxs, n_cutouts, n_objects, obj_h, obj_w,
+                                           cut_h, cut_w, make_float2(0.f, 0.f), stream);
+}
+
+int phx_set_cutouts_f32(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream) {
+  return launch_cutout<float, CutOp::Set>(obj, (void*)cutouts, start_idxs, n_cutouts, n_objects, obj_h, obj_w,
+                                          cut_h, cut_w, 0.f, stream);
+}
+
+int phx_set_cutouts_c64(void* obj, const void* cutouts, const int64_t* start_idxs, int64_t n_cutouts,
+                        int64_t n_objects, int64_t obj_h, int64_t obj_w, int64_t cut_h, int64_t cut_w,
+                        hipStream_t stream) {
+  return launch_cutout<float2, CutOp::Set>(obj, (void*)cutouts, start_idxs, n_cutouts, n_objects, obj_h, obj_w,
+                                           cut_h, cut_w, make_float2(0.f, 0.f), stream);
+}
+
+int phx_fill_u32(void* ptr, uint32_t value, int64_t count, hipStream_t stream) {
+  if (!ptr || count < 0) return PHX_EINVAL;
+  if (count == 0) return PHX_OK;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_u32_k, dim3((unsigned)blocks), dim3(256), 0, stream, (uint32_t*)ptr, value, count);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int inverse, const void* twiddle,
+                 hipStream_t stream) {
+  if (!in || !out || !twiddle || batch < 0 || batch > 0x7fffffffLL) return PHX_EINVAL;
+  if (in == out) return PHX_EINVAL;
+  if (batch == 0) return PHX_OK;
+  switch (n) {
+    case 16: return launch_fft2<16, 256>(in, out, batch, inverse, twiddle, stream);
+    case 32: return launch_fft2<32, 256>(in, out, batch, inverse, twiddle, stream);
+    case 64: return launch_fft2<64, 256>(in, out, batch, inverse, twiddle, stream);
+    case 128: return launch_fft2<128, 512>(in, out, batch, inverse, twiddle, stream);
+    default: return PHX_EUNSUPPORTED;
+  }
+}
+
+int phx_scramble(const void* in, void* out, int64_t batch, int64_t n, int is_complex, int unscramble,
+                 hipStream_t stream) {
+  if (!in || !out || in == out || batch < 0 || batch > 65535) return PHX_EINVAL;
+  if (batch == 0) return PHX_OK;
+  switch (n) {
+    case 16: return launch_scramble<16>(in, out, batch, is_complex, unscramble, stream);
+    case 32: return launch_scramble<32>(in, out, batch, is_complex, unscramble, stream);
+    case 64: return launch_scramble<64>(in, out, batch, is_complex, unscramble, stream);
+    case 128: return launch_scramble<128>(in, out, batch, is_complex, unscramble, stream);
+    default: return PHX_EUNSUPPORTED;
+  }
+}
+
+int phx_scale_c64(void* ptr, int64_t count, float scale, hipStream_t stream) {
+  if (!ptr || count < 0) return PHX_EINVAL;
+  if (count == 0) return PHX_OK;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(scale_c64_k, dim3((unsigned)blocks), dim3(256), 0, stream, (float2*)ptr, count, scale);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,842 @@
+// Fused plane kernels for one group step of phaser's conventional engines (LSQ-ML and
+// ePIE) on gfx950.  One workgroup owns one (position, mode) plane of n x n complex64 in
+// LDS; the object cutout gather, the subpixel-shift filter, the slice transmission, the
+// Fresnel propagator, the modulus constraint and the update reductions are applied
+// while the plane passes through the FFT stages (phx_fft.h), so the stored waves are
+// read and written once per slice step.
+//
+// Reference: phaser/engines/conventional/solvers.py:131-290 (LSQ-ML), :386-504 (ePIE),
+// phaser/engines/common/simulation.py:241-306, noise_models.py:47-63.
+#include "phx_common.h"
+#include "phx_fft.h"
+#include "../../include/phaser_mi355x.h"
+
+#include <climits>
+
+using namespace phx;
+
+namespace {
+
+constexpr float TWO_PI = 6.283185307179586f;
+
+struct RedLayout {
+  int num_obj, den_obj, numP, denP, norm, pos_num, pos_den, total;
+};
+__host__ __device__ inline RedLayout red_layout(int S, int bmax, int M) {
+  RedLayout r;
+  int o = 0;
+  r.num_obj = o; o += S * bmax;
+  r.den_obj = o; o += S;
+  r.numP = o;    o += bmax * M;
+  r.denP = o;    o += 1;
+  r.norm = o;    o += bmax;
+  r.pos_num = o; o += 2 * bmax;
+  r.pos_den = o; o += 2;
+  r.total = o;
+  return r;
+}
+
+enum Param { P_BETA_OBJ = 0, P_BETA_PROBE, P_REG_OBJ, P_REG_PROBE, P_GAMMA, P_OFFSET, P_EPS };
+
+struct Cut {
+  int y0, x0, H, W;
+  __device__ __forceinline__ bool inb(int y, int x) const {
+    const int gy = y0 + y, gx = x0 + x;
+    return gy >= 0 && gy < H && gx >= 0 && gx < W;
+  }
+  __device__ __forceinline__ int idx(int y, int x) const { return (y0 + y) * W + (x0 + x); }
+};
+
+__device__ __forceinline__ Cut cut_of(const phx_group_desc& d, int p) {
+  return Cut{d.start[2 * p], d.start[2 * p + 1], d.obj_h, d.obj_w};
+}
+
+// Object value under the cutout; out of bounds gives 1 (cupy default, object.py:509).
+__device__ __forceinline__ float2 obj_at(const float2* o, const Cut& c, int y, int x) {
+  return c.inb(y, x) ? o[c.idx(y, x)] : make_float2(1.f, 0.f);
+}
+
+// exp(-2 pi i (sx * kx + sy * ky)) in float32 (optics.py:151-170)
+__device__ __forceinline__ float2 shift_filter(float sy, float sx, float ky, float kx) {
+  const float th = -TWO_PI * (sx * kx + sy * ky);
+  float sn, cs;
+  sincosf(th, &sn, &cs);
+  return make_float2(cs, sn);
+}
+
+#define PHX_PLANE_SETUP(N, T)                                                 \
+  __shared__ float2 lds_[Plane<N, T>::LDS_FLOAT2 + N];                        \
+  __shared__ double red_lds_[T / 64];                                         \
+  Plane<N, T> pl{lds_, lds_ + Plane<N, T>::LDS_FLOAT2};                       \
+  load_twiddles<N, T>(lds_ + Plane<N, T>::LDS_FLOAT2, (const float2*)d.twiddle); \
+  __syncthreads();
+
+__device__ __forceinline__ bool slot_active(const phx_group_desc& d, int b) { return b < *d.group_size; }
+
+// ------------------------------------------------------------------------------------
+template <int N, int T>
+__global__ void __launch_bounds__(T) probe_spectra_k(phx_group_desc d) {
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  const int m = blockIdx.x;
+  const float2* P = (const float2*)d.probe + (size_t)m * NN;
+  float2* Pk = (float2*)d.probe_k + (size_t)m * NN;
+  pl.forward([&](int y, int x) { return P[y * N + x]; },
+             [&](int py, int px, float2& v) { Pk[py * N + px] = v; });
+}
+
+// gom = sum_b |prod_s O_s|^2 over the group's cutouts (solvers.py:208).
+__global__ void __launch_bounds__(256) group_obj_mag_k(phx_group_desc d) {
+  const int n = d.n, NN = n * n;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= NN) return;
+  const int y = e / n, x = e % n;
+  const int gs = *d.group_size;
+  const float2* obj = (const float2*)d.object;
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  float acc = 0.f;
+  for (int b = 0; b < gs; ++b) {
+    const Cut c = cut_of(d, d.group_pos[b]);
+    float2 prod = obj_at(obj, c, y, x);
+    for (int s = 1; s < d.n_slices; ++s) prod = cmul(prod, obj_at(obj + s * HW, c, y, x));
+    acc += abs2f(prod);
+  }
+  d.gom[e] = acc;
+}
+
+// psi0 = ifft2(fft2(P) * filt_b), illumination, norm, optional slice-0 step.
+template <int N, int T>
+__global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum, int propagate) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, bmax = d.bmax;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+  const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
+  float2* psi0 = (float2*)d.psi + ((size_t)b * M + m) * NN;
+  const float2* obj0 = (const float2*)d.object;
+  float* gpm0 = d.gpm;
+  double nrm = 0.0;
+
+  pl.cols_iA([&](int py, int px) { return cmul(Pk[py * N + px], shift_filter(sy, sx, d.ky_s[py], d.kx_s[px])); });
+  __syncthreads();
+  pl.cols_iB();
+  __syncthreads();
+  pl.rows_iA();
+  __syncthreads();
+  pl.rows_iB([&](int y, int x, float2 v) {
+    v = cscale(v, inv_nn);
+    psi0[y * N + x] = v;
+    const float a = abs2f(v);
+    nrm += a;
+    const bool inb = c.inb(y, x);
+    if (illum && inb) atomic_add_f32(&gpm0[c.idx(y, x)], a);
+    if (propagate) pl.at(y, x) = cmul(v, inb ? obj0[c.idx(y, x)] : make_float2(1.f, 0.f));
+  });
+  const double tot = block_sum_f64(nrm, red_lds_);
+  if (threadIdx.x == 0) {
+    RedLayout L = red_layout(d.n_slices, bmax, M);
+    atomic_add_f64(&d.red[L.norm + b], tot);
+  }
+  if (!propagate) return;
+  const float2* prop = (const float2*)d.props;
+  float2* psi1 = (float2*)d.psi + (((size_t)bmax + b) * M + m) * NN;
+  pl.convolve([&](int y, int x) { return pl.at(y, x); },
+              [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); },
+              [&](int y, int x, float2 v) { psi1[y * N + x] = cscale(v, inv_nn); });
+}
+
+template <int N, int T>
+__global__ void __launch_bounds__(T) forward_slice_k(phx_group_desc d, int i, int illum) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, bmax = d.bmax;
+  const Cut c = cut_of(d, d.group_pos[b]);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const float2* psi_i = (const float2*)d.psi + (((size_t)i * bmax + b) * M + m) * NN;
+  float2* psi_o = (float2*)d.psi + (((size_t)(i + 1) * bmax + b) * M + m) * NN;
+  const float2* obj_i = (const float2*)d.object + i * HW;
+  float* gpm_i = d.gpm + i * HW;
+  const float2* prop = (const float2*)d.props + (size_t)i * NN;
+  pl.convolve(
+      [&](int y, int x) {
+        const float2 v = psi_i[y * N + x];
+        const bool inb = c.inb(y, x);
+        if (illum && inb) atomic_add_f32(&gpm_i[c.idx(y, x)], abs2f(v));
+        return cmul(v, inb ? obj_i[c.idx(y, x)] : make_float2(1.f, 0.f));
+      },
+      [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); },
+      [&](int y, int x, float2 v) { psi_o[y * N + x] = cscale(v, inv_nn); });
+}
+
+// Exit wave spectrum and model intensity (sum over modes).
+template <int N, int T>
+__global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d, int illum) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, bmax = d.bmax, s = d.n_slices - 1;
+  const Cut c = cut_of(d, d.group_pos[b]);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const float2* psi_s = (const float2*)d.psi + (((size_t)s * bmax + b) * M + m) * NN;
+  const float2* obj_s = (const float2*)d.object + s * HW;
+  float* gpm_s = d.gpm + s * HW;
+  float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
+  float* I = d.intensity + (size_t)b * NN;
+  const bool single = (M == 1);
+  pl.forward(
+      [&](int y, int x) {
+        const float2 v = psi_s[y * N + x];
+        const bool inb = c.inb(y, x);
+        if (illum && inb) atomic_add_f32(&gpm_s[c.idx(y, x)], abs2f(v));
+        return cmul(v, inb ? obj_s[c.idx(y, x)] : make_float2(1.f, 0.f));
+      },
+      [&](int py, int px, float2& v) {
+        const int e = py * N + px;
+        F[e] = v;
+        if (single) I[e] = abs2f(v) * inv_nn;
+        else atomic_add_f32(&I[e], abs2f(v) * inv_nn);
+      });
+}
+
+// Modulus constraint + inverse transform (+ error), noise_models.py:47-63, solvers.py:237-240.
+template <int N, int T>
+__global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int calc_error) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes;
+  const int p = d.group_pos[b];
+  const float2* F = (const float2*)d.farfield + ((size_t)b * M + m) * NN;
+  const float* I = d.intensity + (size_t)b * NN;
+  const float* D = d.patterns_s + (size_t)p * NN;
+  const float* mask = d.mask_s;
+  const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
+  float2* chi = (float2*)d.chi_a + ((size_t)b * M + m) * NN;
+  const bool err = calc_error && m == 0;
+  double e2 = 0.0;
+  pl.inverse(
+      [&](int py, int px) {
+        const int e = py * N + px;
+        const float Iv = I[e], Dv = D[e];
+        if (err) {
+          const float df = Iv - Dv;
+          const float sq = df * df;
+          if (!isnan(sq)) e2 += (double)sq;
+        }
+        const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
+        float u = sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f;
+        u *= mask[e];
+        return cscale(F[e], u);
+      },
+      [&](int y, int x, float2 v) { chi[y * N + x] = cscale(v, inv_nn); });
+  if (err) {
+    const double tot = block_sum_f64(e2, red_lds_);
+    if (threadIdx.x == 0) d.errors[b] = (float)sqrt(tot);
+  }
+}
+
+__device__ __forceinline__ float2* chi_in_buf(const phx_group_desc& d, int i) {
+  return (float2*)(((d.n_slices - 1 - i) % 2 == 0) ? d.chi_a : d.chi_b);
+}
+__device__ __forceinline__ float2* chi_out_buf(const phx_group_desc& d, int i) {
+  return (float2*)(((d.n_slices - 1 - i) % 2 == 0) ? d.chi_b : d.chi_a);
+}
+
+// LSQ-ML backward step at slice i (solvers.py:242-271).
+template <int N, int T>
+__global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, int upd_obj, int upd_probe) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, bmax = d.bmax;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const float2* chi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
+  float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
+  const float2* psi_i = (const float2*)d.psi + (((size_t)i * bmax + b) * M + m) * NN;
+  const float2* obj_i = (const float2*)d.object + i * HW;
+  float2* dO = (float2*)d.d_obj;
+  const bool probe_step = (i == 0) && upd_probe;
+  double numO = 0.0, denO = 0.0, numP = 0.0, denP = 0.0;
+
+  auto src = [&](int y, int x) -> float2 {
+    const float2 cv = chi[y * N + x];
+    const bool inb = c.inb(y, x);
+    const float2 o = inb ? obj_i[c.idx(y, x)] : make_float2(1.f, 0.f);
+    if (upd_obj) {
+      const float2 ps = psi_i[y * N + x];
+      const float2 dOv = cmulc(cv, ps);  // chi * conj(psi)
+      const float2 t = cmul(dOv, ps);
+      numO += (double)(cv.x * t.x + cv.y * t.y);
+      denO += (double)abs2f(t);
+      if (inb) atomic_add_c64(&dO[c.idx(y, x)], dOv);
+    }
+    const float2 dP = cmulc(cv, o);  // chi * conj(O)
+    if (probe_step) {
+      const float2 t2 = cmul(dP, o);
+      numP += (double)(cv.x * t2.x + cv.y * t2.y);
+      denP += (double)abs2f(t2);
+    }
+    return dP;
+  };
+
+  if (i > 0) {
+    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
+                [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); });
+  } else if (probe_step) {
+    const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+    float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+    pl.forward(src, [&](int py, int px, float2& v) {
+      atomic_add_c64(&dPk[py * N + px], cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px])));
+    });
+  } else {
+    for (int e = threadIdx.x; e < NN; e += T) (void)src(e / N, e % N);
+  }
+
+  const RedLayout L = red_layout(d.n_slices, bmax, M);
+  if (upd_obj) {
+    const double a = block_sum_f64(numO, red_lds_);
+    const double bb = block_sum_f64(denO, red_lds_);
+    if (threadIdx.x == 0) {
+      atomic_add_f64(&d.red[L.num_obj + i * bmax + b], a);
+      atomic_add_f64(&d.red[L.den_obj + i], bb);
+    }
+  }
+  if (probe_step) {
+    const double a = block_sum_f64(numP, red_lds_);
+    const double bb = block_sum_f64(denP, red_lds_);
+    if (threadIdx.x == 0) {
+      d.red[L.numP + b * M + m] = a;
+      atomic_add_f64(&d.red[L.denP], bb);
+    }
+  }
+}
+
+// O_i += beta (sum_b alpha_b) d_obj/(gpm+reg) * gpm/(gpm+eps); d_obj = 0 (solvers.py:251-257).
+__global__ void __launch_bounds__(256) lsqml_object_apply_k(phx_group_desc d, int i) {
+  __shared__ float sA;
+  const RedLayout L = red_layout(d.n_slices, d.bmax, d.n_modes);
+  if (threadIdx.x < 64) {
+    const int gs = *d.group_size;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < gs; b += 64) s += d.red[L.num_obj + i * d.bmax + b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0) {
+      const double den = d.red[L.den_obj + i] + (double)d.params[P_GAMMA];
+      sA = (float)((double)d.params[P_BETA_OBJ] * (s / den));
+    }
+  }
+  __syncthreads();
+  const float A = sA, reg = d.params[P_REG_OBJ];
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  float2* obj_i = (float2*)d.object + i * HW;
+  const float* gpm_i = d.gpm + i * HW;
+  float2* dO = (float2*)d.d_obj;
+  bool bad = false;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < HW; e += (size_t)gridDim.x * 256) {
+    const float g = gpm_i[e];
+    const float2 q = cscale(dO[e], 1.0f / (g + reg));
+    const float2 u = cscale(cscale(cscale(q, A), g), 1.0f / (g + 1e-16f));
+    float2 o = obj_i[e];
+    o = cadd(o, u);
+    obj_i[e] = o;
+    dO[e] = make_float2(0.f, 0.f);
+    bad |= !(isfinite(o.x) && isfinite(o.y));
+  }
+  if (bad) atomicMin(d.finite_flag, d.group_index);
+}
+
+// P += beta_P (sum_b alpha_P) ifft2(sum_b fft2(dP) conj(filt))/(gom+reg) * gom/(gom+eps).
+template <int N, int T>
+__global__ void __launch_bounds__(T) lsqml_probe_apply_k(phx_group_desc d) {
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  __shared__ float sA;
+  const int m = blockIdx.x, M = d.n_modes;
+  const RedLayout L = red_layout(d.n_slices, d.bmax, M);
+  if (threadIdx.x < 64) {
+    const int gs = *d.group_size;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < gs; b += 64) s += d.red[L.numP + b * M + m];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0) {
+      const double den = d.red[L.denP] + (double)d.params[P_GAMMA];
+      sA = (float)((double)d.params[P_BETA_PROBE] * (s / den));
+    }
+  }
+  __syncthreads();
+  const float A = sA, reg = d.params[P_REG_PROBE];
+  float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+  float2* P = (float2*)d.probe + (size_t)m * NN;
+  bool bad = false;
+  pl.inverse(
+      [&](int py, int px) {
+        const int e = py * N + px;
+        const float2 v = dPk[e];
+        dPk[e] = make_float2(0.f, 0.f);
+        return v;
+      },
+      [&](int y, int x, float2 v) {
+        const int e = y * N + x;
+        const float g = d.gom[e];
+        const float2 avg = cscale(cscale(v, inv_nn), 1.0f / (g + reg));
+        const float2 u = cscale(cscale(cscale(avg, A), g), 1.0f / (g + 1e-16f));
+        const float2 o = cadd(P[e], u);
+        P[e] = o;
+        bad |= !(isfinite(o.x) && isfinite(o.y));
+      });
+  if (bad) atomicMin(d.finite_flag, d.group_index);
+}
+
+// ---------------------------------- ePIE ------------------------------------------
+// aux[b] = max_px |O_i|^2, aux[bmax + b] = max_px sum_m |psi_i|^2 (solvers.py:462-472)
+__global__ void __launch_bounds__(256) epie_maxima_k(phx_group_desc d, int i) {
+  __shared__ float scratch[4];
+  const int b = blockIdx.x;
+  if (!slot_active(d, b)) return;
+  const int n = d.n, NN = n * n, M = d.n_modes;
+  const Cut c = cut_of(d, d.group_pos[b]);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const float2* obj_i = (const float2*)d.object + i * HW;
+  const float2* psi_i = (const float2*)d.psi + ((size_t)i * d.bmax + b) * M * NN;
+  float mo = 0.f, mp = 0.f;
+  for (int e = threadIdx.x; e < NN; e += 256) {
+    const int y = e / n, x = e % n;
+    mo = fmaxf(mo, abs2f(obj_at(obj_i, c, y, x)));
+    float s = 0.f;
+    for (int mm = 0; mm < M; ++mm) s += abs2f(psi_i[(size_t)mm * NN + e]);
+    mp = fmaxf(mp, s);
+  }
+  mo = block_max_f32(mo, scratch);
+  mp = block_max_f32(mp, scratch);
+  if (threadIdx.x == 0) {
+    d.aux[b] = mo;
+    d.aux[d.bmax + b] = mp;
+  }
+}
+
+template <int N, int T>
+__global__ void __launch_bounds__(T) epie_backward_k(phx_group_desc d, int i, int upd_obj, int upd_probe) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, bmax = d.bmax;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const float2* chi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
+  float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
+  const float2* psi_i = (const float2*)d.psi + (((size_t)i * bmax + b) * M + m) * NN;
+  const float2* obj_i = (const float2*)d.object + i * HW;
+  float2* dO = (float2*)d.d_obj;
+  const float maxO = d.aux[b], maxPsi = d.aux[bmax + b];
+  const float betaS = d.params[P_BETA_OBJ] / (float)d.n_slices;
+
+  auto src = [&](int y, int x) -> float2 {
+    const float2 cv = chi[y * N + x];
+    const bool inb = c.inb(y, x);
+    const float2 o = inb ? obj_i[c.idx(y, x)] : make_float2(1.f, 0.f);
+    if (upd_obj && inb) {
+      const float2 go = cscale(cscale(cmulc(cv, psi_i[y * N + x]), betaS), 1.0f / maxPsi);
+      atomic_add_c64(&dO[c.idx(y, x)], go);
+    }
+    return cscale(cmulc(cv, o), 1.0f / maxO);
+  };
+  if (i > 0) {
+    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
+                [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); });
+  } else if (upd_probe) {
+    const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+    float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+    pl.forward(src, [&](int py, int px, float2& v) {
+      atomic_add_c64(&dPk[py * N + px], cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px])));
+    });
+  } else {
+    for (int e = threadIdx.x; e < NN; e += T) (void)src(e / N, e % N);
+  }
+}
+
+__global__ void __launch_bounds__(256) epie_object_apply_k(phx_group_desc d, int i) {
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  float2* obj_i = (float2*)d.object + i * HW;
+  float2* dO = (float2*)d.d_obj;
+  bool bad = false;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < HW; e += (size_t)gridDim.x * 256) {
+    const float2 o = cadd(obj_i[e], dO[e]);
+    obj_i[e] = o;
+    dO[e] = make_float2(0.f, 0.f);
+    bad |= !(isfinite(o.x) && isfinite(o.y));
+  }
+  if (bad) atomicMin(d.finite_flag, d.group_index);
+}
+
+template <int N, int T>
+__global__ void __launch_bounds__(T) epie_probe_apply_k(phx_group_desc d) {
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int m = blockIdx.x;
+  const float scale = d.params[P_BETA_PROBE] * inv_nn / (float)(*d.group_size);
+  float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+  float2* P = (float2*)d.probe + (size_t)m * NN;
+  bool bad = false;
+  pl.inverse(
+      [&](int py, int px) {
+        const int e = py * N + px;
+        const float2 v = dPk[e];
+        dPk[e] = make_float2(0.f, 0.f);
+        return v;
+      },
+      [&](int y, int x, float2 v) {
+        const int e = y * N + x;
+        const float2 o = cadd(P[e], cscale(v, scale));
+        P[e] = o;
+        bad |= !(isfinite(o.x) && isfinite(o.y));
+      });
+  if (bad) atomicMin(d.finite_flag, d.group_index);
+}
+
+// ----------------------------- dry run / position ---------------------------------
+__global__ void __launch_bounds__(256) dry_run_stats_k(phx_group_desc d) {
+  __shared__ double scratch[4];
+  const int b = blockIdx.x;
+  if (!slot_active(d, b)) return;
+  const int NN = d.n * d.n, M = d.n_modes, s = d.n_slices - 1;
+  const int p = d.group_pos[b];
+  const float2* psi_s = (const float2*)d.psi + ((size_t)s * d.bmax + b) * M * NN;
+  const float* D = d.patterns_s + (size_t)p * NN;
+  double model = 0.0, ex = 0.0;
+  for (int e = threadIdx.x; e < NN; e += 256) {
+    for (int mm = 0; mm < M; ++mm) model += (double)abs2f(psi_s[(size_t)mm * NN + e]);
+    ex += (double)(D[e] * d.mask_s[e]);
+  }
+  model = block_sum_f64(model, scratch);
+  ex = block_sum_f64(ex, scratch);
+  if (threadIdx.x == 0) d.errors[b] = (float)ex / (float)model;
+}
+
+template <int N, int T>
+__global__ void __launch_bounds__(T) position_gradient_k(phx_group_desc d, int dir) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  const int M = d.n_modes, bmax = d.bmax;
+  const RedLayout L = red_layout(d.n_slices, bmax, M);
+  const Cut c = cut_of(d, d.group_pos[b]);
+  const float2* psi0 = (const float2*)d.psi + ((size_t)b * M + m) * NN;
+  const float2* chi = (const float2*)((d.n_slices % 2 == 1) ? d.chi_a : d.chi_b) + ((size_t)b * M + m) * NN;
+  const float2* obj0 = (const float2*)d.object;
+  const float scale = (float)(1.0 / ((double)NN * d.red[L.norm + b]));
+  double num = 0.0, den = 0.0;
+  pl.convolve([&](int y, int x) { return psi0[y * N + x]; },
+              [&](int py, int px, float2 v) {
+                const float k = dir == 0 ? d.ky_s[py] : d.kx_s[px];
+                const float w = TWO_PI * k;
+                return make_float2(w * v.y, -w * v.x);  // v * (-2 pi i k)
+              },
+              [&](int y, int x, float2 v) {
+                const float2 prod = cmul(cscale(v, scale), obj_at(obj0, c, y, x));
+                const float2 cv = chi[y * N + x];
+                num += (double)(cv.x * prod.x + cv.y * prod.y);
+                den += (double)abs2f(prod);
+              });
+  const double a = block_sum_f64(num, red_lds_);
+  const double bb = block_sum_f64(den, red_lds_);
+  if (threadIdx.x == 0) {
+    atomic_add_f64(&d.red[L.pos_num + 2 * b + dir], a);
+    atomic_add_f64(&d.red[L.pos_den + dir], bb);
+  }
+}
+
+// out[pos[b]][natural k] = intensity[b][scrambled k]  (model intensity of each slot)
+template <int N>
+__global__ void exit_intensity_copy_k(phx_group_desc d, float* out) {
+  const int b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= N * N) return;
+  const int py = e / N, px = e % N;
+  const int fe = scr_to_freq<N>(py) * N + scr_to_freq<N>(px);
+  out[(size_t)d.group_pos[b] * N * N + fe] = d.intensity[(size_t)b * N * N + e];
+}
+
+// pos_update[p] = num[b] / den (solvers.py:280), scattered to the group's positions.
+__global__ void position_finish_k(phx_group_desc d, float* pos_update) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= *d.group_size) return;
+  const RedLayout L = red_layout(d.n_slices, d.bmax, d.n_modes);
+  const int p = d.group_pos[b];
+  for (int dir = 0; dir < 2; ++dir)
+    pos_update[2 * p + dir] = (float)(d.red[L.pos_num + 2 * b + dir] / d.red[L.pos_den + dir]);
+}
+
+__global__ void zero4_k(float* a, int64_t na, float* b, int64_t nb, double* c, int64_t nc, float* e, int64_t ne) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < na + nb + nc + ne; k += stride) {
+    if (k < na) a[k] = 0.f;
+    else if (k < na + nb) b[k - na] = 0.f;
+    else if (k < na + nb + nc) c[k - na - nb] = 0.0;
+    else e[k - na - nb - nc] = 0.f;
+  }
+}
+
+// --------------------------------- host side ---------------------------------------
+bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128; }
+
+int check_desc(const phx_group_desc* d) {
+  if (!d || !supported_n(d->n) || d->n_modes < 1 || d->n_slices < 1 || d->bmax < 1 || d->obj_h < 1 ||
+      d->obj_w < 1 || d->n_pos < 1)
+    return PHX_EINVAL;
+  if (!d->group_pos || !d->group_size || !d->start || !d->shift || !d->twiddle || !d->object || !d->probe ||
+      !d->params || !d->psi || !d->red)
+    return PHX_EINVAL;
+  if (d->n_slices > 1 && !d->props) return PHX_EINVAL;
+  return PHX_OK;
+}
+
+#define PHX_PLANE_GRID(d) dim3((unsigned)(d)->n_modes, (unsigned)(d)->bmax)
+
+#define PHX_DISPATCH(n, LAUNCH)       \
+  switch (n) {                        \
+    case 16: LAUNCH(16, 256); break;  \
+    case 32: LAUNCH(32, 256); break;  \
+    case 64: LAUNCH(64, 256); break;  \
+    case 128: LAUNCH(128, 512); break;\
+    default: return PHX_EUNSUPPORTED; \
+  }
+
+}  // namespace
+
+extern "C" {
+
+int64_t phx_red_doubles(int32_t n_slices, int32_t bmax, int32_t n_modes) {
+  return red_layout(n_slices, bmax, n_modes).total;
+}
+
+int phx_group_begin(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->gpm || !d->intensity || !d->errors) return PHX_EINVAL;
+  const int64_t na = (int64_t)d->n_slices * d->obj_h * d->obj_w;
+  const int64_t nb = (int64_t)d->bmax * d->n * d->n;
+  const int64_t nc = red_layout(d->n_slices, d->bmax, d->n_modes).total;
+  const int64_t ne = d->bmax;
+  int64_t blocks = (na + nb + nc + ne + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(zero4_k, dim3((unsigned)blocks), dim3(256), 0, stream, d->gpm, na, d->intensity, nb, d->red, nc,
+                     d->errors, ne);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_probe_spectra(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->probe_k) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((probe_spectra_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_group_obj_mag(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->gom) return PHX_EINVAL;
+  const int NN = d->n * d->n;
+  hipLaunchKernelGGL(group_obj_mag_k, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, stream, *d);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_shift_probes(const phx_group_desc* d, int illum, int propagate, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->probe_k || (illum && !d->gpm)) return PHX_EINVAL;
+  if (propagate && d->n_slices < 2) return PHX_EINVAL;
+#define L(N, T) \
+  hipLaunchKernelGGL((shift_probes_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, illum, propagate)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (slice < 1 || slice > d->n_slices - 2 || (illum && !d->gpm)) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((forward_slice_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, illum)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a || (illum && !d->gpm) ||
+      (calc_error && !d->errors))
+    return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, illum)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+#define L(N, T) \
+  hipLaunchKernelGGL((farfield_update_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
+                       hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (update_object && !d->d_obj) ||
+      (update_probe && !d->d_probe))
+    return PHX_EINVAL;
+#define L(N, T)                                                                                            \
+  hipLaunchKernelGGL((lsqml_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, update_object, \
+                     update_probe)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_object_apply(const phx_group_desc* d, int slice, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (slice < 0 || slice >= d->n_slices || !d->d_obj || !d->gpm || !d->finite_flag) return PHX_EINVAL;
+  const int64_t HW = (int64_t)d->obj_h * d->obj_w;
+  int64_t blocks = (HW + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(lsqml_object_apply_k, dim3((unsigned)blocks), dim3(256), 0, stream, *d, slice);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->d_probe || !d->gom || !d->finite_flag) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((lsqml_probe_apply_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
+                      hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || !d->aux || (update_object && !d->d_obj) ||
+      (update_probe && !d->d_probe))
+    return PHX_EINVAL;
+  hipLaunchKernelGGL(epie_maxima_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d, slice);
+  PHX_CHECK_LAUNCH();
+#define L(N, T)                                                                                           \
+  hipLaunchKernelGGL((epie_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, update_object, \
+                     update_probe)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_epie_object_apply(const phx_group_desc* d, int slice, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (slice < 0 || slice >= d->n_slices || !d->d_obj || !d->finite_flag) return PHX_EINVAL;
+  const int64_t HW = (int64_t)d->obj_h * d->obj_w;
+  int64_t blocks = (HW + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(epie_object_apply_k, dim3((unsigned)blocks), dim3(256), 0, stream, *d, slice);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_epie_probe_apply(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->d_probe || !d->finite_flag) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((epie_probe_apply_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_dry_run_stats(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->patterns_s || !d->mask_s || !d->errors) return PHX_EINVAL;
+  hipLaunchKernelGGL(dry_run_stats_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_position_gradient(const phx_group_desc* d, int dir, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if ((dir != 0 && dir != 1) || !d->chi_a || !d->chi_b) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((position_gradient_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, dir)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_exit_intensity(const phx_group_desc* d, float* out, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!out || !d->farfield || !d->intensity) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  const dim3 grid((unsigned)((d->n * d->n + 255) / 256), (unsigned)d->bmax);
+#define L(N, T) hipLaunchKernelGGL((exit_intensity_copy_k<N>), grid, dim3(256), 0, stream, *d, out)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_position_finish(const phx_group_desc* d, void* pos_update, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!pos_update) return PHX_EINVAL;
+  hipLaunchKernelGGL(position_finish_k, dim3((unsigned)((d->bmax + 63) / 64)), dim3(64), 0, stream, *d,
+                     (float*)pos_update);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+}  // extern "C"

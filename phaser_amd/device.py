@@ -1,0 +1,244 @@
+"""Device workspace for the group step: HBM-resident constants, patterns and buffers, one
+`phx_group_desc` per group slot, and the launch sequences of the LSQ-ML and ePIE group
+steps and their dry runs.
+
+HBM layout (config with P positions, n x n planes, M modes, S slices, H x W slices, B slots):
+  object (S, H, W) c64 and probe (M, n, n) c64 are the state tensors themselves;
+  patterns_s (P, n*n) f32 and mask_s (n*n) f32 are resident, in scrambled frequency order;
+  props_s (S-1, n*n) c64 scrambled; psi (S, B, M, n, n) c64 holds every slice's forward
+  wave for the backward pass; chi_a / chi_b (B, M, n, n) ping-pong the back-propagated
+  exit-wave update; farfield (B, M, n*n) c64 and intensity (B, n*n) f32 carry the exit
+  spectra between the two far-field kernels; gpm (S, H, W) f32, d_obj (H, W) c64,
+  d_probe (M, n*n) c64 are the scatter accumulators; red holds the FP64 step-size sums.
+Per-iteration tables: group_pos (G, B) int32, group_size (G,) int32, start (P, 2) int32,
+shift (P, 2) f32.  Nothing is allocated after construction, so a whole iteration can be
+captured in one HIP graph and replayed.
+"""
+import ctypes
+
+import numpy
+
+from . import _lib
+from .optics import scr_to_freq, twiddles
+
+INT32_MAX = 2**31 - 1
+
+
+def _require_torch_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the MI355X backend needs a visible GPU (no CPU fallback)")
+    return torch
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def to_device_scrambled(torch, arr, n, is_complex, device):
+    """Upload (batch, n, n) natural-order frequency data and scramble it on the device."""
+    arr = numpy.ascontiguousarray(arr)
+    batch = arr.reshape(-1, n, n).shape[0]
+    src = torch.from_numpy(arr.reshape(batch, n * n)).to(device)
+    out = torch.empty_like(src)
+    st = stream_handle()
+    for b0 in range(0, batch, 65535):
+        b1 = min(batch, b0 + 65535)
+        _lib.call("phx_scramble", ctypes.c_void_p(src[b0:b1].data_ptr()), ctypes.c_void_p(out[b0:b1].data_ptr()),
+                  b1 - b0, n, int(is_complex), 0, st)
+    del src
+    return out
+
+
+class GroupWorkspace:
+    """Buffers and descriptors for one engine run over a fixed set of positions."""
+
+    def __init__(self, state, patterns, pattern_mask, props, *, noise_offset, noise_eps, n_groups, bmax,
+                 device='cuda'):
+        torch = _require_torch_gpu()
+        _lib.load()
+        self.torch = torch
+        self.device = device
+        obj, probe = state.object.data, state.probe.data
+        assert obj.dtype == torch.complex64 and probe.dtype == torch.complex64
+        assert obj.is_contiguous() and probe.is_contiguous()
+        self.S, self.H, self.W = (int(v) for v in obj.shape)
+        self.M, ny, nx = (int(v) for v in probe.shape)
+        if ny != nx or ny not in (16, 32, 64, 128):
+            raise NotImplementedError(f"probe shape {ny}x{nx}: the LDS kernels support square 16..128 planes")
+        self.n = n = ny
+        self.NN = n * n
+        pats = numpy.asarray(patterns)
+        self.scan_shape = tuple(pats.shape[:-2])
+        self.P = int(numpy.prod(self.scan_shape))
+        self.G, self.B = int(n_groups), int(bmax)
+        f32, c64, i32 = torch.float32, torch.complex64, torch.int32
+        dev = device
+
+        # constants
+        ky, kx = state.probe.sampling.recip_1d(numpy.float32)
+        f = scr_to_freq(n)
+        self.ky_s = torch.from_numpy(numpy.ascontiguousarray(ky[f])).to(dev)
+        self.kx_s = torch.from_numpy(numpy.ascontiguousarray(kx[f])).to(dev)
+        self.twiddle = torch.from_numpy(twiddles(n)).to(dev)
+        self.patterns_s = to_device_scrambled(torch, pats.astype(numpy.float32), n, False, dev)
+        self.mask_s = to_device_scrambled(torch, numpy.asarray(pattern_mask, dtype=numpy.float32), n, False, dev).reshape(-1)
+        self.props_s = None if props is None else to_device_scrambled(torch, props.astype(numpy.complex64), n, True, dev)
+        self.params = torch.zeros(8, dtype=f32, device=dev)
+        self.params_host = numpy.zeros(8, dtype=numpy.float32)
+        self.params_host[5] = noise_offset
+        self.params_host[6] = noise_eps
+
+        # per-iteration tables
+        self.group_pos = torch.zeros((self.G, self.B), dtype=i32, device=dev)
+        self.group_size = torch.zeros(self.G, dtype=i32, device=dev)
+        self.start = torch.zeros((self.P, 2), dtype=i32, device=dev)
+        self.shift = torch.zeros((self.P, 2), dtype=f32, device=dev)
+
+        # work buffers
+        S, B, M, NN, H, W = self.S, self.B, self.M, self.NN, self.H, self.W
+        self.psi = torch.empty((S, B, M, NN), dtype=c64, device=dev)
+        self.chi_a = torch.empty((B, M, NN), dtype=c64, device=dev)
+        self.chi_b = torch.empty((B, M, NN), dtype=c64, device=dev)
+        self.farfield = torch.empty((B, M, NN), dtype=c64, device=dev)
+        self.intensity = torch.empty((B, NN), dtype=f32, device=dev)
+        self.gpm = torch.empty((S, H, W), dtype=f32, device=dev)
+        self.d_obj = torch.zeros((H, W), dtype=c64, device=dev)
+        self.d_probe = torch.zeros((M, NN), dtype=c64, device=dev)
+        self.probe_k = torch.empty((M, NN), dtype=c64, device=dev)
+        self.gom = torch.empty(NN, dtype=f32, device=dev)
+        nred = int(_lib.load().phx_red_doubles(S, B, M))
+        self.red = torch.zeros((self.G, nred), dtype=torch.float64, device=dev)
+        self.errors = torch.zeros((self.G, B), dtype=f32, device=dev)
+        self.aux = torch.zeros(2 * B, dtype=f32, device=dev)
+        self.finite_flag = torch.full((1,), INT32_MAX, dtype=i32, device=dev)
+        self.pos_update = torch.zeros((self.P, 2), dtype=f32, device=dev)
+
+        self.object = obj
+        self.probe = probe
+        self.descs = [self._make_desc(g) for g in range(self.G)]
+        self.desc_refs = [ctypes.byref(d) for d in self.descs]
+
+    def _make_desc(self, g):
+        d = _lib.GroupDesc()
+        d.n, d.n_modes, d.n_slices, d.obj_h, d.obj_w = self.n, self.M, self.S, self.H, self.W
+        d.bmax, d.group_index, d.n_pos = self.B, g, self.P
+        d.group_pos = self.group_pos[g].data_ptr()
+        d.group_size = self.group_size[g:g + 1].data_ptr()
+        d.start = self.start.data_ptr()
+        d.shift = self.shift.data_ptr()
+        d.ky_s, d.kx_s, d.twiddle = self.ky_s.data_ptr(), self.kx_s.data_ptr(), self.twiddle.data_ptr()
+        d.object, d.probe = self.object.data_ptr(), self.probe.data_ptr()
+        d.props = None if self.props_s is None else self.props_s.data_ptr()
+        d.patterns_s, d.mask_s, d.params = self.patterns_s.data_ptr(), self.mask_s.data_ptr(), self.params.data_ptr()
+        d.psi, d.chi_a, d.chi_b = self.psi.data_ptr(), self.chi_a.data_ptr(), self.chi_b.data_ptr()
+        d.farfield, d.intensity, d.gpm = self.farfield.data_ptr(), self.intensity.data_ptr(), self.gpm.data_ptr()
+        d.d_obj, d.d_probe, d.probe_k = self.d_obj.data_ptr(), self.d_probe.data_ptr(), self.probe_k.data_ptr()
+        d.gom = self.gom.data_ptr()
+        d.red = self.red[g].data_ptr()
+        d.errors = self.errors[g].data_ptr()
+        d.aux = self.aux.data_ptr()
+        d.finite_flag = self.finite_flag.data_ptr()
+        return d
+
+    # ------------------------------------------------------------------ host tables
+    def set_positions(self, obj_sampling, scan):
+        """Start indices / subpixel shifts for every position (FP64 host math, uploaded)."""
+        scan = numpy.asarray(scan).reshape(-1, 2)
+        start, shift = obj_sampling.cutout_indices(scan, (self.n, self.n))
+        if numpy.any(start < 0) or numpy.any(start[:, 0] + self.n > self.H) or numpy.any(start[:, 1] + self.n > self.W):
+            raise ValueError("probe cutout extends outside the object (the numpy backend asserts start >= 0, "
+                             "object.py:318); pad the object")
+        self.start.copy_(self.torch.from_numpy(start.astype(numpy.int32)))
+        self.shift.copy_(self.torch.from_numpy(shift.astype(numpy.float32)))
+
+    def set_groups(self, groups):
+        """Upload the iteration's groups (list of index arrays into the scan shape)."""
+        if len(groups) != self.G:
+            raise ValueError(f"expected {self.G} groups, got {len(groups)}")
+        table = numpy.zeros((self.G, self.B), dtype=numpy.int32)
+        sizes = numpy.zeros(self.G, dtype=numpy.int32)
+        for g, grp in enumerate(groups):
+            flat = numpy.ravel_multi_index(tuple(numpy.asarray(grp)), self.scan_shape)
+            if len(flat) > self.B:
+                raise ValueError(f"group of {len(flat)} exceeds the {self.B} slots")
+            table[g, :len(flat)] = flat
+            sizes[g] = len(flat)
+        self.group_pos.copy_(self.torch.from_numpy(table))
+        self.group_size.copy_(self.torch.from_numpy(sizes))
+        self.sizes_host = sizes
+        self.table_host = table
+
+    def set_params(self, beta_object=0.0, beta_probe=0.0, reg_object=0.0, reg_probe=0.0, gamma=0.0):
+        self.params_host[:5] = (beta_object, beta_probe, reg_object, reg_probe, gamma)
+        self.params.copy_(self.torch.from_numpy(self.params_host))
+
+    def reset_finite_flag(self):
+        _lib.call("phx_fill_u32", _ptr(self.finite_flag), INT32_MAX, 1, stream_handle())
+
+    def finite_group(self):
+        v = int(self.finite_flag.item())
+        return None if v == INT32_MAX else v
+
+    # ------------------------------------------------------------------ launches
+    def launch_dry_run(self, g):
+        d, st = self.desc_refs[g], stream_handle()
+        c = _lib.call
+        c("phx_group_begin", d, st)
+        c("phx_probe_spectra", d, st)
+        c("phx_shift_probes", d, 0, int(self.S > 1), st)
+        for i in range(1, self.S - 1):
+            c("phx_forward_slice", d, i, 0, st)
+        c("phx_dry_run_stats", d, st)
+
+    def launch_forward(self, g, calc_error, illum=True):
+        d, st = self.desc_refs[g], stream_handle()
+        c = _lib.call
+        c("phx_group_begin", d, st)
+        c("phx_probe_spectra", d, st)
+        c("phx_shift_probes", d, int(illum), int(self.S > 1), st)
+        for i in range(1, self.S - 1):
+            c("phx_forward_slice", d, i, int(illum), st)
+        c("phx_farfield", d, int(illum and self.S > 1), int(calc_error), st)
+
+    def launch_lsqml_group(self, g, *, update_object, update_probe, update_position, calc_error):
+        """lsqml_run (solvers.py:171-290) for group slot g."""
+        d, st = self.desc_refs[g], stream_handle()
+        c = _lib.call
+        self.launch_forward(g, calc_error, illum=True)
+        if update_probe:
+            c("phx_group_obj_mag", d, st)
+        for i in range(self.S - 1, -1, -1):
+            c("phx_lsqml_backward", d, i, int(update_object), int(update_probe), st)
+            if update_object:
+                c("phx_lsqml_object_apply", d, i, st)
+        if update_probe:
+            c("phx_lsqml_probe_apply", d, st)
+        if update_position:
+            c("phx_position_gradient", d, 0, st)
+            c("phx_position_gradient", d, 1, st)
+            c("phx_position_finish", d, _ptr(self.pos_update), st)
+
+    def launch_epie_group(self, g, *, update_object, update_probe, update_position, calc_error):
+        """epie_run (solvers.py:415-504) for group slot g."""
+        d, st = self.desc_refs[g], stream_handle()
+        c = _lib.call
+        self.launch_forward(g, calc_error, illum=False)
+        for i in range(self.S - 1, -1, -1):
+            c("phx_epie_backward", d, i, int(update_object), int(update_probe), st)
+            if update_object:
+                c("phx_epie_object_apply", d, i, st)
+        if update_probe:
+            c("phx_epie_probe_apply", d, st)
+        if update_position:
+            c("phx_position_gradient", d, 0, st)
+            c("phx_position_gradient", d, 1, st)
+            c("phx_position_finish", d, _ptr(self.pos_update), st)
+
+    def scale_probe(self, factor):
+        _lib.call("phx_scale_c64", _ptr(self.probe), self.probe.numel(), float(factor), stream_handle())

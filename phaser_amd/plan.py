@@ -1,0 +1,248 @@
+"""Engine plans with the reference's field names and defaults (phaser/plan.py:43-173,
+phaser/hooks/solver.py:79-98, phaser/types.py:125-167, phaser/hooks/schedule.py).
+
+Plans are plain dataclasses; `from_dict` accepts the mapping form of a plan YAML engine
+entry (`{'type': 'conventional', 'solver': 'lsqml', 'noise_model': 'amplitude', ...}`)."""
+from dataclasses import dataclass, field, fields
+import typing as t
+
+import numpy
+
+
+@dataclass(frozen=True)
+class Flag:
+    """Iteration flag (types.py:125-146): true when i > after, i < before and
+    (i - after) % every == 0."""
+    after: int = 0
+    every: int = 1
+    before: t.Optional[int] = None
+
+    def __call__(self, args) -> bool:
+        i = args['state'].iter.engine_iter
+        return ((self.before is None or i < self.before) and i > self.after
+                and (i - self.after) % self.every == 0)
+
+
+def process_flag(flag):
+    """bool or Flag -> callable(args) -> bool (types.py:149-160)."""
+    if isinstance(flag, bool):
+        return lambda _args: flag
+    if isinstance(flag, dict):
+        flag = Flag(**flag)
+    return flag
+
+
+def process_schedule(schedule):
+    """float or callable(args) -> float (types.py:163-167); dict forms of the known
+    schedules (constant / piecewise / expr, hooks/schedule.py:33-75) are accepted."""
+    if isinstance(schedule, (int, float)):
+        return lambda _args: float(schedule)
+    if isinstance(schedule, dict):
+        kind = schedule.get('type')
+        if kind == 'constant':
+            v = float(schedule['value'])
+            return lambda _args: v
+        if kind == 'piecewise':
+            init = process_schedule(schedule['init'])
+            steps = sorted(((int(k), process_schedule(v)) for k, v in schedule['steps'].items()), reverse=True)
+
+            def piecewise(args):
+                i = args['state'].iter.engine_iter
+                for thresh, val in steps:
+                    if i >= thresh:
+                        return val(args)
+                return init(args)
+            return piecewise
+        if kind == 'expr':
+            expr = schedule['expr']
+            return lambda args: float(eval(expr, {  # same evaluation namespace as schedule.py:60-71
+                'i': args['state'].iter.engine_iter, 'iter': args['state'].iter,
+                'state': args['state'], 'niter': args['niter'], 'np': numpy}))
+        raise ValueError(f"Unknown schedule {schedule!r}")
+    return schedule
+
+
+@dataclass
+class AmplitudeNoisePlan:
+    gaussian_variance: float = 0.1
+    eps: float = 1.0e-3
+    offset: float = 0.0
+    name: str = 'amplitude'
+
+
+@dataclass
+class AnscombeNoisePlan(AmplitudeNoisePlan):
+    offset: float = 0.375
+    name: str = 'anscombe'
+
+
+@dataclass
+class PoissonNoisePlan(AmplitudeNoisePlan):
+    eps: float = 1.0e-3
+    name: str = 'poisson'
+
+
+NOISE_MODELS = {'amplitude': AmplitudeNoisePlan, 'anscombe': AnscombeNoisePlan, 'poisson': PoissonNoisePlan}
+
+
+@dataclass
+class LSQMLSolverPlan:
+    stochastic: bool = True
+    beta_object: t.Any = 1.0
+    beta_probe: t.Any = 1.0
+    illum_reg_object: t.Any = 1e-2
+    illum_reg_probe: t.Any = 1e-2
+    gamma: t.Any = 1e-4
+    name: str = 'lsqml'
+
+
+@dataclass
+class EPIESolverPlan:
+    beta_object: t.Any = 1.0
+    beta_probe: t.Any = 1.0
+    name: str = 'epie'
+
+
+SOLVERS = {'lsqml': LSQMLSolverPlan, 'epie': EPIESolverPlan}
+
+
+@dataclass
+class SteepestDescentPositionSolverPlan:
+    step_size: float = 1e-2
+    max_step_size: t.Optional[float] = None
+    name: str = 'steepest_descent'
+
+
+@dataclass
+class MomentumPositionSolverPlan:
+    step_size: float = 1e-2
+    max_step_size: t.Optional[float] = None
+    momentum: float = 0.9
+    name: str = 'momentum'
+
+
+POSITION_SOLVERS = {'steepest_descent': SteepestDescentPositionSolverPlan, 'momentum': MomentumPositionSolverPlan}
+
+
+@dataclass
+class EnginePlan:
+    """phaser/plan.py:43-83"""
+    sim_shape: t.Optional[t.Tuple[int, int]] = None
+    resize_method: str = 'pad_crop'
+    probe_modes: int = 1
+    base_mode_power: float = 0.7
+    bwlim_frac: t.Optional[float] = 2 / 3
+    obj_pad_px: float = 5.0
+    slices: t.Any = None
+    niter: int = 10
+    grouping: t.Optional[int] = None
+    compact: bool = False
+    shuffle_groups: t.Any = None
+    buffer_n_groups: int = 2
+    update_probe: t.Any = True
+    update_object: t.Any = True
+    update_positions: t.Any = False
+    update_tilt: t.Any = False
+    calc_error: t.Any = field(default_factory=lambda: Flag(every=1))
+    calc_error_fraction: float = 0.1
+    save: t.Any = False
+    save_images: t.Any = False
+    early_termination: t.Optional[int] = None
+    early_termination_smoothing: float = 0.9
+    send_every_group: bool = False
+
+
+@dataclass
+class ConventionalEnginePlan(EnginePlan):
+    """phaser/plan.py:126-132.  `group_constraints` / `iter_constraints` must be empty
+    lists here (constraint kernels are not part of this backend)."""
+    noise_model: t.Any = None
+    solver: t.Any = None
+    position_solver: t.Any = None
+    group_constraints: t.List[t.Any] = field(default_factory=list)
+    iter_constraints: t.List[t.Any] = field(default_factory=list)
+
+    def __post_init__(self):
+        self.noise_model = _resolve(self.noise_model, NOISE_MODELS, 'noise_model')
+        self.solver = _resolve(self.solver, SOLVERS, 'solver')
+        if self.position_solver is not None:
+            self.position_solver = _resolve(self.position_solver, POSITION_SOLVERS, 'position_solver')
+
+
+@dataclass
+class SGDSolverPlan:
+    learning_rate: t.Any = 1.0
+    momentum: t.Any = None
+    nesterov: bool = True
+    name: str = 'sgd'
+
+
+@dataclass
+class AdamSolverPlan:
+    learning_rate: t.Any = 1.0
+    b1: float = 0.9
+    b2: float = 0.999
+    eps: float = 1.0e-8
+    eps_root: float = 0.0
+    nesterov: bool = False
+    name: str = 'adam'
+
+
+@dataclass
+class PolyakSGDSolverPlan:
+    max_learning_rate: t.Any = 1.0
+    f_min: float = 0.0
+    scaling: t.Any = 1.0
+    eps: float = 0.0
+    name: str = 'polyak_sgd'
+
+
+GRADIENT_SOLVERS = {'sgd': SGDSolverPlan, 'adam': AdamSolverPlan, 'polyak_sgd': PolyakSGDSolverPlan}
+
+
+@dataclass
+class GradientEnginePlan(EnginePlan):
+    """phaser/plan.py:135-141.  `solvers` maps a frozenset of variable names
+    ('object', 'probe', 'positions', 'tilt') to a solver plan."""
+    noise_model: t.Any = None
+    solvers: t.Dict[t.Any, t.Any] = field(default_factory=dict)
+    regularizers: t.List[t.Any] = field(default_factory=list)
+    group_constraints: t.List[t.Any] = field(default_factory=list)
+    iter_constraints: t.List[t.Any] = field(default_factory=list)
+
+    def __post_init__(self):
+        self.noise_model = _resolve(self.noise_model, NOISE_MODELS, 'noise_model')
+        solvers = {}
+        for k, v in self.solvers.items():
+            key = frozenset([k]) if isinstance(k, str) else frozenset(k)
+            solvers[key] = _resolve(v, GRADIENT_SOLVERS, 'solver')
+        self.solvers = solvers
+
+
+def _resolve(val, known, what):
+    if val is None:
+        raise ValueError(f"'{what}' must be specified")
+    if isinstance(val, str):
+        if val not in known:
+            raise ValueError(f"Unknown {what} '{val}'")
+        return known[val]()
+    if isinstance(val, dict):
+        d = dict(val)
+        kind = d.pop('type')
+        cls = known[kind]
+        names = {f.name for f in fields(cls)}
+        bad = set(d) - names
+        if bad:
+            raise ValueError(f"Unknown {what} options {sorted(bad)}")
+        return cls(**d)
+    return val
+
+
+def from_dict(d: t.Mapping[str, t.Any]):
+    """Engine plan from the mapping form of a plan YAML `engines:` entry."""
+    d = dict(d)
+    kind = d.pop('type', 'conventional')
+    if 'calc_error' in d and isinstance(d['calc_error'], dict):
+        d['calc_error'] = Flag(**d['calc_error'])
+    cls = {'conventional': ConventionalEnginePlan, 'gradient': GradientEnginePlan}[kind]
+    return cls(**d)
