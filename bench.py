@@ -1,0 +1,233 @@
+"""Benchmark: multislice LSQ-ML on one MI355X (HIP backend) vs the numpy oracle on the host.
+
+Workload (the 8-slice LSQ-ML configuration): object 8 slices on the 517x517 reconstruction
+grid of a 64x64 raster scan (step 6 px, 0.2 A/px, 512x512 nominal), slice thickness 20 A;
+probe 128x128 with 4 modes (300 kV, 20 mrad, 100 A defocus); 4096 patterns of 128x128
+float32 with Poisson noise at 1e5 e-/pattern; amplitude noise model, groups of 64, position
+correction off, LSQ-ML defaults (beta 1, illum_reg 1e-2, gamma 1e-4).
+
+A step is one LSQ-ML iteration over all 4096 patterns (64 group steps), so
+value = 4096 * steps / elapsed [patterns*iter/s].  Patterns, object and probe are resident
+in HBM before the timed region; the presolve and `--warmup` iterations are untimed.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-graphs] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy
+
+METRIC = "patterns·iter/s; time to NRMSE 0.05, learned-init vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, B, M, S, NN, HW):
+    """Algorithmic HBM bytes of one launch of each group kernel (B positions, c = 8 B,
+    r = 4 B): the per-pattern terms of the LSQ-ML inter-slice dataflow model."""
+    c, r = 8, 4
+    per_pos = {
+        # read psi_i (M planes), read O_i cutout, illumination read-modify-write, write psi_{i+1}
+        'forward_slice': 2 * M * c * NN + c * NN + 2 * r * NN,
+        # read chi, psi_i, O_i; write back-propagated chi; d_obj scatter read-modify-write
+        'lsqml_backward': 3 * M * c * NN + c * NN + 2 * c * NN,
+        # shifted probes: write psi0, illumination; slice-0 step reads O_0, writes psi1
+        'shift_probes': 2 * M * c * NN + c * NN + 2 * r * NN,
+        # exit wave: read psi, O, write spectrum, intensity r-m-w
+        'farfield_fwd': 2 * M * c * NN + c * NN + 2 * r * NN,
+        # modulus constraint: read spectrum, intensity, pattern; write chi
+        'farfield_update': 2 * M * c * NN + r * NN + r * NN,
+    }
+    return B * per_pos[name]
+
+
+def lsqml_bytes_per_pattern_iter(S, M, NN, HW, B):
+    """Algorithmic bytes per pattern-iteration of the LSQ-ML dataflow:
+    2SMcN + 2SrN + 2ScN + rN + McN + (3S-1)McN + 2ScN + S*H*W*(3c+r)/B."""
+    c, r = 8, 4
+    return (2 * S * M * c * NN + 2 * S * r * NN + 2 * S * c * NN + r * NN + M * c * NN
+            + (3 * S - 1) * M * c * NN + 2 * S * c * NN + S * HW * (3 * c + r) / B)
+
+
+def time_kernels(ws, groups, torch, n_groups):
+    """Average device duration of each kernel family over one non-graph iteration, from
+    HIP events recorded on the stream the kernels are launched on (torch's current
+    stream)."""
+    from phaser_amd import _lib
+    from phaser_amd.device import stream_handle
+    durations = {}
+
+    def timed(name, fn, *args):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _lib.call(fn, *args)
+        e1.record()
+        durations.setdefault(name, []).append((e0, e1))
+
+    for g in range(n_groups):
+        d, st = ws.desc_refs[g], stream_handle()
+        _lib.call("phx_group_begin", d, st)
+        _lib.call("phx_probe_spectra", d, st)
+        timed('shift_probes', "phx_shift_probes", d, 1, int(ws.S > 1), st)
+        for i in range(1, ws.S - 1):
+            timed('forward_slice', "phx_forward_slice", d, i, 1, st)
+        timed('farfield', "phx_farfield", d, int(ws.S > 1), 0, st)
+        _lib.call("phx_group_obj_mag", d, st)
+        for i in range(ws.S - 1, -1, -1):
+            timed('lsqml_backward' if i > 0 else 'lsqml_backward_slice0', "phx_lsqml_backward", d, i, 1, 1, st)
+            timed('lsqml_object_apply', "phx_lsqml_object_apply", d, i, st)
+        timed('lsqml_probe_apply', "phx_lsqml_probe_apply", d, st)
+    torch.cuda.synchronize()
+    out = {}
+    for k, evs in durations.items():
+        ms = [a.elapsed_time(b) for a, b in evs]
+        out[k] = {'launches': len(ms), 'avg_us': 1e3 * float(numpy.mean(ms)), 'total_ms': float(numpy.sum(ms))}
+    return out
+
+
+def cpu_baseline(state, pats, groups, niter_groups=1):
+    """The FP32 numpy oracle's LSQ-ML group step (solvers.py:171-290 restated) on a
+    bounded sample: `niter_groups` groups of 64 patterns of this workload, 1 thread."""
+    import oracle.engine  # noqa: F401  (the oracle is the timed CPU baseline here)
+    import oracle.noise_models as onoise
+    from oracle.optics import make_propagators
+    from oracle.solvers import Sim, lsqml_run
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from tests.helpers import to_oracle_state
+    ost = to_oracle_state(state)
+    sim = Sim(ost, onoise.AmplitudeNoiseModel(), numpy.float32)
+    props = make_propagators(sim.ky, sim.kx, ost.probe.sampling.k_max, ost.wavelength, ost.object.thicknesses)
+    n = 0
+    t0 = time.perf_counter()
+    for g in groups[:niter_groups]:
+        lsqml_run(sim, g, pats.patterns[tuple(g)], pattern_mask=pats.pattern_mask, props=props, beta_object=1.0,
+                  beta_probe=1.0, update_object=True, update_probe=True, update_position=False,
+                  calc_error=False, illum_reg_object=1e-2, illum_reg_probe=1e-2, gamma=1e-4)
+        n += g.shape[-1]
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-groups", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
+              "build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    from phaser_amd.engines.conventional import LSQMLSolver, SimulationState
+    from phaser_amd.device import GroupWorkspace
+    from phaser_amd.groups import GroupManager
+    from phaser_amd.optics import make_propagators
+    from phaser_amd.plan import ConventionalEnginePlan
+    from phaser_amd.synthetic import CONFIG_2, make_dataset
+
+    cfg = CONFIG_2
+    log("generating synthetic data (GPU forward model + Poisson noise)")
+    t0 = time.perf_counter()
+    state0, pats, truth = make_dataset(cfg)
+    log(f"data ready in {time.perf_counter() - t0:.1f} s: patterns {pats.patterns.shape}, object {state0.object.data.shape}")
+
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=args.warmup + args.steps,
+                                  grouping=64)
+    state = state0.to_device()
+    groups = GroupManager(state.scan, plan.grouping, False, seed=0)
+    props = make_propagators(state.probe.sampling, state.wavelength, state.object.thicknesses, plan.bwlim_frac)
+    G = len(groups)
+    n_pos = int(numpy.prod(state.scan.shape[:-1]))
+    B = int(numpy.ceil(n_pos / G))
+    ws = GroupWorkspace(state, pats.patterns, pats.pattern_mask, props, noise_offset=0.0, noise_eps=1e-3,
+                        n_groups=G, bmax=B)
+    ws.set_positions(state.object.sampling, state.scan)
+    sim = SimulationState(state=state, noise_model=plan.noise_model, workspace=ws)
+    solver = LSQMLSolver(plan, plan.solver, use_graphs=not args.no_graphs)
+    log("presolve")
+    solver.presolve(sim, groups.iter(state.scan), patterns=None, pattern_mask=None, propagators=None)
+    torch.cuda.synchronize()
+    mask = numpy.zeros(G, dtype=bool)
+    mask[::max(1, G // 7)] = True
+
+    def iteration(i):
+        state.iter.engine_iter = i
+        return solver.run_iteration(sim, groups.iter(state.scan, i, True), patterns=None, pattern_mask=None,
+                                    propagators=None, update_object=True, update_probe=True, update_positions=False,
+                                    calc_error=True, calc_error_mask=mask, observer=_NullObs())
+
+    log(f"warmup x{args.warmup}")
+    for i in range(1, args.warmup + 1):
+        iteration(i)
+    torch.cuda.synchronize()
+    log(f"timed x{args.steps}")
+    t0 = time.perf_counter()
+    errs = []
+    for i in range(args.warmup + 1, args.warmup + args.steps + 1):
+        _s, _p, e = iteration(i)
+        errs.append(float(numpy.nanmean(numpy.concatenate(e))))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = n_pos * args.steps / elapsed
+    log(f"{value:.0f} patterns*iter/s, {1e3 * elapsed / args.steps:.2f} ms/iter, errors {errs[0]:.4g} -> {errs[-1]:.4g}")
+
+    # kernel durations (HIP events on the launch stream), one extra non-graph iteration
+    ws.set_groups(list(groups.iter(state.scan, 999, True)))
+    kt = time_kernels(ws, groups.iter, torch, G)
+    dom = max((k for k in kt if k in ('forward_slice', 'lsqml_backward', 'shift_probes')),
+              key=lambda k: kt[k]['total_ms'])
+    NN, M, S, HW = ws.NN, ws.M, ws.S, ws.H * ws.W
+    alg = kernel_bytes(dom, B, M, S, NN, HW)
+    achieved = alg / (kt[dom]['avg_us'] * 1e-6) / 1e9
+    per_pi = lsqml_bytes_per_pattern_iter(S, M, NN, HW, B)
+    whole = per_pi * value / 1e9
+    log(f"kernels: {json.dumps(kt)}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (numpy oracle, 1 thread)")
+        cgroups = list(GroupManager(state0.scan, 64, False, seed=0).iter(state0.scan))
+        rate, npat, dt = cpu_baseline(state0, pats, cgroups, args.cpu_groups)
+        cpu = {"value": rate, "unit": "patterns·iter/s", "cores": 1, "kind": "port",
+               "sample": f"{npat} patterns x 1 LSQ-ML group step ({args.cpu_groups} group(s) of this workload) "
+                         f"with the FP32 numpy oracle on the host, {dt:.1f} s; host has {os.cpu_count()} cores"}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "patterns·iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "multislice LSQ-ML: 8 slices of 517x517 (512x512 nominal) complex64, 128x128 probe "
+                               "x 4 modes, 4096 patterns 128x128 f32 (Poisson 1e5), groups of 64, 1 step = 1 "
+                               "iteration over all patterns", "graphs": not args.no_graphs},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg, "avg_launch_us": kt[dom]['avg_us'],
+                     "whole_iteration_alg_GBs": whole, "whole_iteration_frac": whole / HBM_PEAK_GBS},
+        "cpu_baseline": cpu,
+        "detector_error_first_last": [errs[0], errs[-1]],
+    }
+    print(json.dumps(line), flush=True)
+
+
+class _NullObs:
+    def update_group(self, *a, **k):
+        pass
+
+
+if __name__ == "__main__":
+    main()

@@ -215,14 +215,19 @@ class GroupWorkspace:
             c("phx_group_obj_mag", d, st)
         for i in range(self.S - 1, -1, -1):
             c("phx_lsqml_backward", d, i, int(update_object), int(update_probe), st)
+            if i == 0 and update_position:
+                # the position step uses the group's pre-update slice-0 cutout and the
+                # final chi (solvers.py:275-288), so it runs before slice 0 is updated
+                self._launch_position(d, st)
             if update_object:
                 c("phx_lsqml_object_apply", d, i, st)
         if update_probe:
             c("phx_lsqml_probe_apply", d, st)
-        if update_position:
-            c("phx_position_gradient", d, 0, st)
-            c("phx_position_gradient", d, 1, st)
-            c("phx_position_finish", d, _ptr(self.pos_update), st)
+
+    def _launch_position(self, d, st):
+        _lib.call("phx_position_gradient", d, 0, st)
+        _lib.call("phx_position_gradient", d, 1, st)
+        _lib.call("phx_position_finish", d, _ptr(self.pos_update), st)
 
     def launch_epie_group(self, g, *, update_object, update_probe, update_position, calc_error):
         """epie_run (solvers.py:415-504) for group slot g."""
@@ -231,14 +236,12 @@ class GroupWorkspace:
         self.launch_forward(g, calc_error, illum=False)
         for i in range(self.S - 1, -1, -1):
             c("phx_epie_backward", d, i, int(update_object), int(update_probe), st)
+            if i == 0 and update_position:
+                self._launch_position(d, st)
             if update_object:
                 c("phx_epie_object_apply", d, i, st)
         if update_probe:
             c("phx_epie_probe_apply", d, st)
-        if update_position:
-            c("phx_position_gradient", d, 0, st)
-            c("phx_position_gradient", d, 1, st)
-            c("phx_position_finish", d, _ptr(self.pos_update), st)
 
     def scale_probe(self, factor):
         _lib.call("phx_scale_c64", _ptr(self.probe), self.probe.numel(), float(factor), stream_handle())
