@@ -111,6 +111,11 @@ typedef struct phx_group_desc {
   float* errors;             /* (bmax) per-slot detector error of this group */
   float* aux;                /* (2*bmax) float32 scratch (ePIE maxima) */
   int32_t* finite_flag;      /* device int: min group index with NaN/inf (INT32_MAX = none) */
+  int32_t psi_slots;         /* planes of psi kept per slot: >= S (or <= 0) stores every slice;
+                                3 keeps psi_0 plus two ping-pong slices (recompute mode) */
+  int32_t pad_;
+  void* grad_obj;            /* (S, H, W) complex64 object gradient accumulator (gradient engine) */
+  void* grad_probe;          /* (M, n, n) complex64 probe gradient (gradient engine) */
 } phx_group_desc;
 
 /* Doubles needed by phx_group_desc.red. */
@@ -168,6 +173,45 @@ int phx_dry_run_stats(const phx_group_desc* d, hipStream_t stream);
 /* Position gradient numerators/denominators (solvers.py:275-288) for direction dir
  * (0 = y, 1 = x), using the final chi (in chi_a when S is odd, chi_b when even). */
 int phx_position_gradient(const phx_group_desc* d, int dir, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * Gradient engine (phaser/engines/gradient/run.py:293-438): hand-written adjoint of the
+ * multislice model.  Gradients follow torch's convention g = 2 dL/dconj(z) (the reference
+ * uses jax's, conj(g), and steps along -conj(g_jax) = -g).
+ * --------------------------------------------------------------------------------- */
+/* Exit spectrum + intensity, then chi_a = dL/d(exit wave) of the amplitude/Anscombe
+ * (poisson = 0, noise_models.py:29-45) or Poisson (poisson = 1, :86-102) loss times
+ * params[8]; the group loss (FP64) is added to the group's red.loss slot. */
+int phx_farfield_gradient(const phx_group_desc* d, int poisson, hipStream_t stream);
+/* Adjoint slice step i (reads the gradient from chi_a/chi_b like phx_lsqml_backward):
+ * object gradient into grad_obj[i]; for i > 0 back-propagates; for i == 0 accumulates
+ * the probe-gradient spectrum in d_probe and the position gradients in red.pos_num. */
+int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, int want_probe, int want_positions,
+                          hipStream_t stream);
+/* grad_probe = ifft2(d_probe) (ortho); d_probe = 0. */
+int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream);
+/* Gradient-engine rescale factors into errors[b]: sum(D) / sum |psi_{S-1} O_{S-1}|^2
+ * (gradient/run.py:408-438: last slice included, pattern mask not applied). */
+int phx_dry_run_stats_gradient(const phx_group_desc* d, hipStream_t stream);
+
+/* Optimizer steps on complex64 parameters (gradient/solvers.py:88-138 over optax): the
+ * direction is d = -grad / scale; finite_flag (may be NULL) records flag_value on NaN/inf.
+ * SGD: t = d + momentum t; u = nesterov ? d + momentum t : t (or u = d without momentum);
+ *      param += lr u. */
+int phx_sgd_step(void* param, const void* grad, void* trace, int64_t n, float scale, float lr, float momentum,
+                 int nesterov, int use_momentum, int32_t* finite_flag, int32_t flag_value, hipStream_t stream);
+/* Adam (optax.scale_by_adam): mu, nu updated in place; count is the step number after
+ * increment (1 on the first step); param += lr mu_hat / (sqrt(nu_hat + eps_root) + eps). */
+int phx_adam_step(void* param, const void* grad, void* mu, float* nu, int64_t n, float scale, float lr, float b1,
+                  float b2, float eps, float eps_root, int32_t count, int nesterov, int32_t* finite_flag,
+                  int32_t flag_value, hipStream_t stream);
+/* acc += sum |grad / scale|^2 (FP64). */
+int phx_sqnorm_accumulate(const void* grad, int64_t n, float scale, double* acc, hipStream_t stream);
+/* Polyak SGD (scale_by_learning_rate(scaling) + optax.scale_by_polyak): step =
+ * min((loss - f_min) / (scaling^2 sqnorm + eps), max_lr); param += step scaling d. */
+int phx_polyak_step(void* param, const void* grad, int64_t n, float scale, float scaling, float max_lr, float f_min,
+                    float eps, const double* sqnorm, const double* loss, int32_t* finite_flag, int32_t flag_value,
+                    hipStream_t stream);
 
 /* Model intensity sum_m |fft2(psi[S-1] * O_{S-1})|^2 of each slot, written in natural
  * (corner-origin) order to out[group_pos[b]] ((P, n*n) float32): the forward model used to

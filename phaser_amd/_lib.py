@@ -30,6 +30,7 @@ class GroupDesc(ctypes.Structure):
         ("psi", c_vp), ("chi_a", c_vp), ("chi_b", c_vp), ("farfield", c_vp), ("intensity", c_vp),
         ("gpm", c_vp), ("d_obj", c_vp), ("d_probe", c_vp), ("probe_k", c_vp), ("gom", c_vp),
         ("red", c_vp), ("errors", c_vp), ("aux", c_vp), ("finite_flag", c_vp),
+        ("psi_slots", c_i32), ("pad_", c_i32), ("grad_obj", c_vp), ("grad_probe", c_vp),
     ]
 
 
@@ -64,6 +65,16 @@ SIGNATURES = {
     "phx_position_gradient": (c_int, [_P, c_int, c_vp]),
     "phx_position_finish": (c_int, [_P, c_vp, c_vp]),
     "phx_exit_intensity": (c_int, [_P, c_vp, c_vp]),
+    "phx_farfield_gradient": (c_int, [_P, c_int, c_vp]),
+    "phx_gradient_backward": (c_int, [_P, c_int, c_int, c_int, c_int, c_vp]),
+    "phx_gradient_probe_finish": (c_int, [_P, c_vp]),
+    "phx_dry_run_stats_gradient": (c_int, [_P, c_vp]),
+    "phx_sgd_step": (c_int, [c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_int, c_int, c_vp, c_i32, c_vp]),
+    "phx_adam_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32,
+                              c_int, c_vp, c_i32, c_vp]),
+    "phx_sqnorm_accumulate": (c_int, [c_vp, c_i64, c_f32, c_vp, c_vp]),
+    "phx_polyak_step": (c_int, [c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_vp, c_vp, c_vp, c_i32,
+                                c_vp]),
 }
 
 _lib = None

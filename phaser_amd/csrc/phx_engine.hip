@@ -20,7 +20,7 @@ namespace {
 constexpr float TWO_PI = 6.283185307179586f;
 
 struct RedLayout {
-  int num_obj, den_obj, numP, denP, norm, pos_num, pos_den, total;
+  int num_obj, den_obj, numP, denP, norm, pos_num, pos_den, loss, total;
 };
 __host__ __device__ inline RedLayout red_layout(int S, int bmax, int M) {
   RedLayout r;
@@ -32,11 +32,17 @@ __host__ __device__ inline RedLayout red_layout(int S, int bmax, int M) {
   r.norm = o;    o += bmax;
   r.pos_num = o; o += 2 * bmax;
   r.pos_den = o; o += 2;
+  r.loss = o;    o += 1;
   r.total = o;
   return r;
 }
 
-enum Param { P_BETA_OBJ = 0, P_BETA_PROBE, P_REG_OBJ, P_REG_PROBE, P_GAMMA, P_OFFSET, P_EPS };
+// params[] slots; P_VAR = 1 + gaussian_variance of the amplitude loss, P_LOSS_SCALE the
+// upstream gradient of the loss (autograd's grad_output)
+enum Param { P_BETA_OBJ = 0, P_BETA_PROBE, P_REG_OBJ, P_REG_PROBE, P_GAMMA, P_OFFSET, P_EPS, P_VAR, P_LOSS_SCALE };
+
+// far-field modes of farfield_update_k
+enum FarMode { FM_MODULUS = 0, FM_GRAD_AMPLITUDE = 1, FM_GRAD_POISSON = 2 };
 
 struct Cut {
   int y0, x0, H, W;
@@ -72,6 +78,15 @@ __device__ __forceinline__ float2 shift_filter(float sy, float sx, float ky, flo
   __syncthreads();
 
 __device__ __forceinline__ bool slot_active(const phx_group_desc& d, int b) { return b < *d.group_size; }
+
+// Offset of the stored wave of slice i, slot b, mode m.  With psi_slots < S only psi_0
+// is kept permanently and slices 1.. alternate between two further planes, so slice i
+// is recomputed from psi_0 when the adjoint needs it.
+__device__ __forceinline__ size_t psi_off(const phx_group_desc& d, int i, int b, int m) {
+  const bool full = d.psi_slots <= 0 || d.psi_slots >= d.n_slices;
+  const int slot = full ? i : (i == 0 ? 0 : 1 + ((i - 1) & 1));
+  return (((size_t)slot * d.bmax + b) * d.n_modes + m) * (size_t)d.n * d.n;
+}
 
 // ------------------------------------------------------------------------------------
 template <int N, int T>
@@ -117,7 +132,7 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum,
   const Cut c = cut_of(d, p);
   const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
   const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
-  float2* psi0 = (float2*)d.psi + ((size_t)b * M + m) * NN;
+  float2* psi0 = (float2*)d.psi + psi_off(d, 0, b, m);
   const float2* obj0 = (const float2*)d.object;
   float* gpm0 = d.gpm;
   double nrm = 0.0;
@@ -144,7 +159,7 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum,
   }
   if (!propagate) return;
   const float2* prop = (const float2*)d.props;
-  float2* psi1 = (float2*)d.psi + (((size_t)bmax + b) * M + m) * NN;
+  float2* psi1 = (float2*)d.psi + psi_off(d, 1, b, m);
   pl.convolve([&](int y, int x) { return pl.at(y, x); },
               [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); },
               [&](int y, int x, float2 v) { psi1[y * N + x] = cscale(v, inv_nn); });
@@ -160,8 +175,8 @@ __global__ void __launch_bounds__(T) forward_slice_k(phx_group_desc d, int i, in
   const int M = d.n_modes, bmax = d.bmax;
   const Cut c = cut_of(d, d.group_pos[b]);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
-  const float2* psi_i = (const float2*)d.psi + (((size_t)i * bmax + b) * M + m) * NN;
-  float2* psi_o = (float2*)d.psi + (((size_t)(i + 1) * bmax + b) * M + m) * NN;
+  const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
+  float2* psi_o = (float2*)d.psi + psi_off(d, i + 1, b, m);
   const float2* obj_i = (const float2*)d.object + i * HW;
   float* gpm_i = d.gpm + i * HW;
   const float2* prop = (const float2*)d.props + (size_t)i * NN;
@@ -187,7 +202,7 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d, int illum)
   const int M = d.n_modes, bmax = d.bmax, s = d.n_slices - 1;
   const Cut c = cut_of(d, d.group_pos[b]);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
-  const float2* psi_s = (const float2*)d.psi + (((size_t)s * bmax + b) * M + m) * NN;
+  const float2* psi_s = (const float2*)d.psi + psi_off(d, s, b, m);
   const float2* obj_s = (const float2*)d.object + s * HW;
   float* gpm_s = d.gpm + s * HW;
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
@@ -208,9 +223,14 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d, int illum)
       });
 }
 
-// Modulus constraint + inverse transform (+ error), noise_models.py:47-63, solvers.py:237-240.
+// Far-field step + inverse transform into chi_a.
+//  FM_MODULUS: amplitude/Anscombe modulus constraint (noise_models.py:47-63) and, when
+//    calc_error, errors[b] = sqrt(nansum((I - D)^2)) (solvers.py:237, FP64 sum);
+//  FM_GRAD_AMPLITUDE / FM_GRAD_POISSON: gradient of the amplitude (noise_models.py:29-45)
+//    or Poisson (:86-102) loss w.r.t. the exit wave, in torch's convention
+//    (2 dL/dconj(a)), times params[P_LOSS_SCALE]; the group loss (FP64) goes to red.loss.
 template <int N, int T>
-__global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int calc_error) {
+__global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int calc_error, int mode) {
   const int m = blockIdx.x, b = blockIdx.y;
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
@@ -224,26 +244,45 @@ __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int cal
   const float* mask = d.mask_s;
   const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
   float2* chi = (float2*)d.chi_a + ((size_t)b * M + m) * NN;
-  const bool err = calc_error && m == 0;
+  const bool err = calc_error && m == 0 && mode == FM_MODULUS;
+  const bool want_loss = mode != FM_MODULUS && m == 0;
+  const float var = d.params[P_VAR];
+  const float gscale = mode != FM_MODULUS ? d.params[P_LOSS_SCALE] : 1.0f;
   double e2 = 0.0;
   pl.inverse(
       [&](int py, int px) {
         const int e = py * N + px;
         const float Iv = I[e], Dv = D[e];
-        if (err) {
-          const float df = Iv - Dv;
-          const float sq = df * df;
-          if (!isnan(sq)) e2 += (double)sq;
-        }
         const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
-        float u = sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f;
-        u *= mask[e];
+        float u;
+        if (mode == FM_MODULUS) {
+          if (err) {
+            const float df = Iv - Dv;
+            const float sq = df * df;
+            if (!isnan(sq)) e2 += (double)sq;
+          }
+          u = sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f;
+          u *= mask[e];
+        } else if (mode == FM_GRAD_AMPLITUDE) {
+          const float si = sqrtf(Iv + off);
+          const float s = sqrtf(Dp + off) - si - eps;
+          if (want_loss) e2 += (double)(mask[e] * s * s);
+          u = -4.0f * mask[e] * s / (var * si) * gscale;
+        } else {
+          if (want_loss) e2 += (double)(mask[e] * (Iv + eps + Dp * (logf(Dp + eps) - logf(Iv + eps) - 1.0f)));
+          u = 2.0f * mask[e] * (1.0f - Dp / (Iv + eps)) * gscale;
+        }
         return cscale(F[e], u);
       },
       [&](int y, int x, float2 v) { chi[y * N + x] = cscale(v, inv_nn); });
   if (err) {
     const double tot = block_sum_f64(e2, red_lds_);
     if (threadIdx.x == 0) d.errors[b] = (float)sqrt(tot);
+  }
+  if (want_loss) {
+    double tot = block_sum_f64(e2, red_lds_);
+    if (mode == FM_GRAD_AMPLITUDE) tot *= 2.0 / (double)var;
+    if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(d.n_slices, d.bmax, M).loss], tot);
   }
 }
 
@@ -268,7 +307,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, i
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float2* chi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
   float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
-  const float2* psi_i = (const float2*)d.psi + (((size_t)i * bmax + b) * M + m) * NN;
+  const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
   const float2* obj_i = (const float2*)d.object + i * HW;
   float2* dO = (float2*)d.d_obj;
   const bool probe_step = (i == 0) && upd_probe;
@@ -415,7 +454,7 @@ __global__ void __launch_bounds__(256) epie_maxima_k(phx_group_desc d, int i) {
   const Cut c = cut_of(d, d.group_pos[b]);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float2* obj_i = (const float2*)d.object + i * HW;
-  const float2* psi_i = (const float2*)d.psi + ((size_t)i * d.bmax + b) * M * NN;
+  const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, 0);
   float mo = 0.f, mp = 0.f;
   for (int e = threadIdx.x; e < NN; e += 256) {
     const int y = e / n, x = e % n;
@@ -445,7 +484,7 @@ __global__ void __launch_bounds__(T) epie_backward_k(phx_group_desc d, int i, in
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float2* chi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
   float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
-  const float2* psi_i = (const float2*)d.psi + (((size_t)i * bmax + b) * M + m) * NN;
+  const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
   const float2* obj_i = (const float2*)d.object + i * HW;
   float2* dO = (float2*)d.d_obj;
   const float maxO = d.aux[b], maxPsi = d.aux[bmax + b];
@@ -523,7 +562,7 @@ __global__ void __launch_bounds__(256) dry_run_stats_k(phx_group_desc d) {
   if (!slot_active(d, b)) return;
   const int NN = d.n * d.n, M = d.n_modes, s = d.n_slices - 1;
   const int p = d.group_pos[b];
-  const float2* psi_s = (const float2*)d.psi + ((size_t)s * d.bmax + b) * M * NN;
+  const float2* psi_s = (const float2*)d.psi + psi_off(d, s, b, 0);
   const float* D = d.patterns_s + (size_t)p * NN;
   double model = 0.0, ex = 0.0;
   for (int e = threadIdx.x; e < NN; e += 256) {
@@ -544,7 +583,7 @@ __global__ void __launch_bounds__(T) position_gradient_k(phx_group_desc d, int d
   const int M = d.n_modes, bmax = d.bmax;
   const RedLayout L = red_layout(d.n_slices, bmax, M);
   const Cut c = cut_of(d, d.group_pos[b]);
-  const float2* psi0 = (const float2*)d.psi + ((size_t)b * M + m) * NN;
+  const float2* psi0 = (const float2*)d.psi + psi_off(d, 0, b, m);
   const float2* chi = (const float2*)((d.n_slices % 2 == 1) ? d.chi_a : d.chi_b) + ((size_t)b * M + m) * NN;
   const float2* obj0 = (const float2*)d.object;
   const float scale = (float)(1.0 / ((double)NN * d.red[L.norm + b]));
@@ -567,6 +606,113 @@ __global__ void __launch_bounds__(T) position_gradient_k(phx_group_desc d, int d
     atomic_add_f64(&d.red[L.pos_num + 2 * b + dir], a);
     atomic_add_f64(&d.red[L.pos_den + dir], bb);
   }
+}
+
+// -------------------------- gradient engine: adjoint ---------------------------------
+// Adjoint of the multislice model at slice i (phaser/engines/gradient/run.py:352-405,
+// differentiated by hand).  chi_in holds g_a = 2 dL/dconj(a_i) for a_i = psi_i * O_i
+// (torch's convention).  Writes g_O += g_a conj(psi_i) into grad_obj[i]; for i > 0 the
+// back-propagated g_{a_{i-1}} = ifft2(fft2(g_a conj(O_i)) conj(prop_{i-1})); for i == 0
+// the probe-gradient spectrum sum_b fft2(g_psi0) conj(filt_b) into d_probe and the
+// position gradients dL/dy_b, dL/dx_b = Re sum conj(fft2 g_psi0) fft2(P) filt (-2 pi i k).
+template <int N, int T>
+__global__ void __launch_bounds__(T) gradient_backward_k(phx_group_desc d, int i, int want_obj, int want_probe,
+                                                         int want_pos) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const float2* g = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
+  float2* g_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
+  const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
+  const float2* obj_i = (const float2*)d.object + i * HW;
+  float2* gO = (float2*)d.grad_obj + i * HW;
+
+  auto src = [&](int y, int x) -> float2 {
+    const float2 gv = g[y * N + x];
+    const bool inb = c.inb(y, x);
+    const float2 o = inb ? obj_i[c.idx(y, x)] : make_float2(1.f, 0.f);
+    if (want_obj && inb) atomic_add_c64(&gO[c.idx(y, x)], cmulc(gv, psi_i[y * N + x]));
+    return cmulc(gv, o);
+  };
+  if (i > 0) {
+    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
+                [&](int y, int x, float2 v) { g_o[y * N + x] = cscale(v, inv_nn); });
+  } else if (want_probe || want_pos) {
+    const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+    const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
+    float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+    double gy = 0.0, gx = 0.0;
+    pl.forward(src, [&](int py, int px, float2& v) {
+      const int e = py * N + px;
+      const float2 f = shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]);
+      if (want_probe) atomic_add_c64(&dPk[e], cmulc(v, f));
+      if (want_pos) {
+        const float2 t = cmulc(cmul(Pk[e], f), v);  // conj(v) * fft2(P) filt
+        gy += (double)(TWO_PI * d.ky_s[py] * t.y);
+        gx += (double)(TWO_PI * d.kx_s[px] * t.y);
+      }
+    });
+    if (want_pos) {
+      const double ty = block_sum_f64(gy, red_lds_);
+      const double tx = block_sum_f64(gx, red_lds_);
+      if (threadIdx.x == 0) {
+        const RedLayout L = red_layout(d.n_slices, d.bmax, M);
+        atomic_add_f64(&d.red[L.pos_num + 2 * b], ty * (double)inv_nn);
+        atomic_add_f64(&d.red[L.pos_num + 2 * b + 1], tx * (double)inv_nn);
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < NN; e += T) (void)src(e / N, e % N);
+  }
+}
+
+// grad_probe[m] = ifft2(d_probe[m]) (natural order, ortho scale); d_probe = 0.
+template <int N, int T>
+__global__ void __launch_bounds__(T) gradient_probe_finish_k(phx_group_desc d) {
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int m = blockIdx.x;
+  float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+  float2* gP = (float2*)d.grad_probe + (size_t)m * NN;
+  pl.inverse(
+      [&](int py, int px) {
+        const int e = py * N + px;
+        const float2 v = dPk[e];
+        dPk[e] = make_float2(0.f, 0.f);
+        return v;
+      },
+      [&](int y, int x, float2 v) { gP[y * N + x] = cscale(v, inv_nn); });
+}
+
+// Gradient-engine dry run (gradient/run.py:408-438): rescale[b] = sum(D) / sum_m
+// |psi_{S-1} O_{S-1}|^2 (last slice included, no mask; Parseval for the fft2).
+__global__ void __launch_bounds__(256) dry_run_stats_gradient_k(phx_group_desc d) {
+  __shared__ double scratch[4];
+  const int b = blockIdx.x;
+  if (!slot_active(d, b)) return;
+  const int n = d.n, NN = n * n, M = d.n_modes, s = d.n_slices - 1;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const float2* psi_s = (const float2*)d.psi + psi_off(d, s, b, 0);
+  const float2* obj_s = (const float2*)d.object + (size_t)s * d.obj_h * d.obj_w;
+  const float* D = d.patterns_s + (size_t)p * NN;
+  double model = 0.0, ex = 0.0;
+  for (int e = threadIdx.x; e < NN; e += 256) {
+    const float2 o = obj_at(obj_s, c, e / n, e % n);
+    for (int mm = 0; mm < M; ++mm) model += (double)abs2f(cmul(psi_s[(size_t)mm * NN + e], o));
+    ex += (double)D[e];
+  }
+  model = block_sum_f64(model, scratch);
+  ex = block_sum_f64(ex, scratch);
+  if (threadIdx.x == 0) d.errors[b] = (float)ex / (float)model;
 }
 
 // out[pos[b]][natural k] = intensity[b][scrambled k]  (model intensity of each slot)
@@ -687,7 +833,7 @@ int phx_shift_probes(const phx_group_desc* d, int illum, int propagate, hipStrea
 int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (slice < 1 || slice > d->n_slices - 2 || (illum && !d->gpm)) return PHX_EINVAL;
+  if (slice < 0 || slice > d->n_slices - 2 || (illum && !d->gpm)) return PHX_EINVAL;
 #define L(N, T) hipLaunchKernelGGL((forward_slice_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, illum)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -706,7 +852,7 @@ int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t
 #undef L
   PHX_CHECK_LAUNCH();
 #define L(N, T) \
-  hipLaunchKernelGGL((farfield_update_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error)
+  hipLaunchKernelGGL((farfield_update_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error, 0)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
@@ -809,6 +955,59 @@ int phx_position_gradient(const phx_group_desc* d, int dir, hipStream_t stream) 
 #define L(N, T) hipLaunchKernelGGL((position_gradient_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, dir)
   PHX_DISPATCH(d->n, L);
 #undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_farfield_gradient(const phx_group_desc* d, int poisson, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  const int mode = poisson ? (int)FM_GRAD_POISSON : (int)FM_GRAD_AMPLITUDE;
+#define L(N, T) hipLaunchKernelGGL((farfield_update_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0, mode)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, int want_probe, int want_positions,
+                          hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (want_object && !d->grad_obj) ||
+      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)))
+    return PHX_EINVAL;
+#define L(N, T)                                                                                              \
+  hipLaunchKernelGGL((gradient_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, want_object, \
+                     want_probe, want_positions)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->d_probe || !d->grad_probe) return PHX_EINVAL;
+#define L(N, T) \
+  hipLaunchKernelGGL((gradient_probe_finish_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_dry_run_stats_gradient(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->patterns_s || !d->errors) return PHX_EINVAL;
+  hipLaunchKernelGGL(dry_run_stats_gradient_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

@@ -59,7 +59,7 @@ class GroupWorkspace:
     """Buffers and descriptors for one engine run over a fixed set of positions."""
 
     def __init__(self, state, patterns, pattern_mask, props, *, noise_offset, noise_eps, n_groups, bmax,
-                 device='cuda'):
+                 noise_var=1.1, psi_slots=0, gradients=False, device='cuda'):
         torch = _require_torch_gpu()
         _lib.load()
         self.torch = torch
@@ -89,10 +89,14 @@ class GroupWorkspace:
         self.patterns_s = to_device_scrambled(torch, pats.astype(numpy.float32), n, False, dev)
         self.mask_s = to_device_scrambled(torch, numpy.asarray(pattern_mask, dtype=numpy.float32), n, False, dev).reshape(-1)
         self.props_s = None if props is None else to_device_scrambled(torch, props.astype(numpy.complex64), n, True, dev)
-        self.params = torch.zeros(8, dtype=f32, device=dev)
-        self.params_host = numpy.zeros(8, dtype=numpy.float32)
+        # [beta_obj, beta_probe, reg_obj, reg_probe, gamma, offset, eps, var, loss_scale]
+        self.params = torch.zeros(16, dtype=f32, device=dev)
+        self.params_host = numpy.zeros(16, dtype=numpy.float32)
         self.params_host[5] = noise_offset
         self.params_host[6] = noise_eps
+        self.params_host[7] = noise_var
+        self.params_host[8] = 1.0
+        self.params.copy_(torch.from_numpy(self.params_host))
 
         # per-iteration tables
         self.group_pos = torch.zeros((self.G, self.B), dtype=i32, device=dev)
@@ -102,7 +106,8 @@ class GroupWorkspace:
 
         # work buffers
         S, B, M, NN, H, W = self.S, self.B, self.M, self.NN, self.H, self.W
-        self.psi = torch.empty((S, B, M, NN), dtype=c64, device=dev)
+        self.psi_slots = S if (psi_slots <= 0 or psi_slots >= S) else 3
+        self.psi = torch.empty((self.psi_slots, B, M, NN), dtype=c64, device=dev)
         self.chi_a = torch.empty((B, M, NN), dtype=c64, device=dev)
         self.chi_b = torch.empty((B, M, NN), dtype=c64, device=dev)
         self.farfield = torch.empty((B, M, NN), dtype=c64, device=dev)
@@ -118,6 +123,8 @@ class GroupWorkspace:
         self.aux = torch.zeros(2 * B, dtype=f32, device=dev)
         self.finite_flag = torch.full((1,), INT32_MAX, dtype=i32, device=dev)
         self.pos_update = torch.zeros((self.P, 2), dtype=f32, device=dev)
+        self.grad_obj = torch.zeros((S, H, W), dtype=c64, device=dev) if gradients else None
+        self.grad_probe = torch.zeros((M, NN), dtype=c64, device=dev) if gradients else None
 
         self.object = obj
         self.probe = probe
@@ -144,6 +151,9 @@ class GroupWorkspace:
         d.errors = self.errors[g].data_ptr()
         d.aux = self.aux.data_ptr()
         d.finite_flag = self.finite_flag.data_ptr()
+        d.psi_slots = self.psi_slots
+        d.grad_obj = None if self.grad_obj is None else self.grad_obj.data_ptr()
+        d.grad_probe = None if self.grad_probe is None else self.grad_probe.data_ptr()
         return d
 
     # ------------------------------------------------------------------ host tables

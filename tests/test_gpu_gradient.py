@@ -1,0 +1,125 @@
+"""GPU parity of the hand-written adjoint (gradient engine) against the oracle's gradient,
+which is itself pinned by finite differences and torch autograd
+(tests/test_oracle_gradient.py); slice recompute vs stored waves; the
+torch.autograd.Function; and the engine loop with SGD / Adam / Polyak against the oracle
+gradient engine."""
+import numpy
+import pytest
+
+import oracle.gradient as og
+from oracle.optics import make_propagators
+from tests.helpers import rel_err, to_oracle_state, oracle_simulate
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(S=3, M=2, n=32, scan=(5, 5), noise='amplitude', seed=21):
+    from phaser_amd.synthetic import SyntheticConfig, make_dataset
+    cfg = SyntheticConfig(n=n, n_modes=M, n_slices=S, scan_shape=scan, scan_step_px=3.0, jitter_px=0.4, seed=seed,
+                          defocus=30.0)
+    state, pats, truth = make_dataset(cfg, simulate=oracle_simulate)
+    # a non-flat current object so every gradient term is generic
+    rng = numpy.random.default_rng(seed)
+    state.object.data = (state.object.data * numpy.exp(0.1j * rng.normal(size=state.object.data.shape))).astype(numpy.complex64)
+    state.probe.data = (state.probe.data * numpy.float32(numpy.sqrt(1e5))).astype(numpy.complex64)
+    return cfg, state, pats
+
+
+def _oracle_grad(state, pats, group, noise_name):
+    st = to_oracle_state(state, dtype=numpy.float64)
+    ky, kx = st.probe.sampling.recip_grid(dtype=numpy.float64)
+    props = make_propagators(ky, kx, st.probe.sampling.k_max, st.wavelength, st.object.thicknesses)
+    nz = {'name': noise_name, 'offset': 0.375 if noise_name == 'anscombe' else 0.0, 'eps': 1e-3,
+          'gaussian_variance': 0.1}
+    return og.model_grad(st, group, pats.patterns.astype(numpy.float64), pats.pattern_mask.astype(numpy.float64),
+                         nz, ky, kx, props)
+
+
+@pytest.mark.parametrize('noise', ['amplitude', 'poisson'])
+@pytest.mark.parametrize('recompute', [False, True])
+def test_adjoint_matches_oracle_gradient(noise, recompute):
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import NOISE_MODELS
+    _cfg, state, pats = _problem(S=4)
+    group = numpy.array(numpy.unravel_index(numpy.arange(25), (5, 5)))
+    model, dstate = make_model(state, pats.patterns, pats.pattern_mask, NOISE_MODELS[noise](), groups=[group],
+                               recompute=recompute)
+    model.forward(0)
+    model.backward(0)
+    loss = float(model.loss_tensor(0).item())
+    g_obj = model.ws.grad_obj.cpu().numpy()
+    g_probe = model.ws.grad_probe.view(model.ws.probe.shape).cpu().numpy()
+    g_pos = model.position_grads(0)[:25].cpu().numpy()
+    o_loss, o_obj, o_probe, o_pos = _oracle_grad(state, pats, group, noise)
+    e = (abs(loss - o_loss) / abs(o_loss), rel_err(g_obj, o_obj), rel_err(g_probe, o_probe), rel_err(g_pos, o_pos))
+    print(f"{noise} recompute={recompute}: loss {e[0]:.2e} object {e[1]:.2e} probe {e[2]:.2e} positions {e[3]:.2e}",
+          flush=True)
+    assert e[0] < 1e-5 and e[1] < 1e-4 and e[2] < 1e-4 and e[3] < 1e-3
+
+
+def test_recompute_equals_store():
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import AmplitudeNoisePlan
+    _cfg, state, pats = _problem(S=6, M=2)
+    group = numpy.array(numpy.unravel_index(numpy.arange(25), (5, 5)))
+    res = []
+    for rc in (False, True):
+        model, _ = make_model(state, pats.patterns, pats.pattern_mask, AmplitudeNoisePlan(), groups=[group],
+                              recompute=rc)
+        model.forward(0)
+        model.backward(0)
+        res.append((model.ws.grad_obj.cpu().numpy(), model.ws.grad_probe.cpu().numpy(),
+                    model.position_grads(0).cpu().numpy(), model.ws.psi.shape[0]))
+    assert res[0][3] == 6 and res[1][3] == 3
+    for a, b in zip(res[0][:3], res[1][:3]):
+        assert rel_err(a, b) < 1e-6
+
+
+def test_autograd_function_backward():
+    import torch
+    from phaser_amd.engines.gradient import make_model, multislice_loss
+    from phaser_amd.plan import AmplitudeNoisePlan
+    _cfg, state, pats = _problem(S=3)
+    group = numpy.array(numpy.unravel_index(numpy.arange(25), (5, 5)))
+    model, dstate = make_model(state, pats.patterns, pats.pattern_mask, AmplitudeNoisePlan(), groups=[group])
+    obj = dstate.object.data.requires_grad_(True)
+    probe = dstate.probe.data.requires_grad_(True)
+    pos = torch.zeros((25, 2), dtype=torch.float32, device='cuda', requires_grad=True)
+    loss = multislice_loss(obj, probe, pos, model, 0)
+    (2.0 * loss).backward()
+    o_loss, o_obj, o_probe, o_pos = _oracle_grad(state, pats, group, 'amplitude')
+    assert abs(float(loss) - o_loss) / o_loss < 1e-5
+    assert rel_err(obj.grad.cpu().numpy(), 2 * o_obj) < 1e-4
+    assert rel_err(probe.grad.cpu().numpy(), 2 * o_probe) < 1e-4
+    assert rel_err(pos.grad.cpu().numpy(), 2 * o_pos) < 1e-3
+
+
+@pytest.mark.parametrize('solver', ['sgd', 'adam', 'polyak_sgd'])
+def test_gradient_engine_matches_oracle(solver):
+    import oracle.state as ost
+    from phaser_amd.engines.gradient import run_engine
+    from phaser_amd.plan import GradientEnginePlan, SGDSolverPlan, AdamSolverPlan, PolyakSGDSolverPlan
+    _cfg, state, pats = _problem(S=2, M=2, scan=(6, 6), seed=23)
+    state.probe.data = (state.probe.data / numpy.float32(numpy.sqrt(1e5))).astype(numpy.complex64)
+    if solver == 'sgd':
+        sp, hp = SGDSolverPlan(learning_rate=0.5, momentum=0.5), dict(learning_rate=0.5, momentum=0.5, nesterov=True)
+    elif solver == 'adam':
+        sp, hp = AdamSolverPlan(learning_rate=1e-3), dict(learning_rate=1e-3)
+    else:
+        sp, hp = PolyakSGDSolverPlan(max_learning_rate=1.0, f_min=0.0, scaling=0.5), \
+            dict(max_learning_rate=1.0, f_min=0.0, scaling=0.5)
+    plan = GradientEnginePlan(noise_model='amplitude', solvers={('object', 'probe'): sp}, niter=3, grouping=12)
+    g = run_engine({'data': pats, 'state': state.copy(), 'seed': 3, 'dtype': numpy.float32}, plan).to_numpy()
+    res = []
+    for dt in (numpy.float32, numpy.float64):
+        o = to_oracle_state(state, dtype=dt)
+        nz = {'name': 'amplitude', 'offset': 0.0, 'eps': 1e-3, 'gaussian_variance': 0.1}
+        og.run_engine(o, pats.patterns.astype(dt), pats.pattern_mask.astype(dt), noise=nz,
+                      solvers=[og.Solver(solver, ['object', 'probe'], **hp)], niter=3, grouping=12, seed=3, dtype=dt)
+        res.append(o)
+    o32, o64 = res
+    for name, f in (('object', lambda s: s.object.data), ('probe', lambda s: s.probe.data),
+                    ('loss', lambda s: s.progress.detector_errors)):
+        a, b, c = rel_err(f(g), f(o32)), rel_err(f(g), f(o64)), rel_err(f(o32), f(o64))
+        print(f"{solver} {name}: gpu-vs-fp32 {a:.2e} gpu-vs-fp64 {b:.2e} fp32-vs-fp64 {c:.2e}", flush=True)
+        assert b <= 3 * c + 1e-4
