@@ -29,22 +29,26 @@ def log(msg):
 
 
 def kernel_bytes(name, B, M, S, NN, HW):
-    """Algorithmic HBM bytes of one launch of each group kernel (B positions, c = 8 B,
-    r = 4 B): the per-pattern terms of the LSQ-ML inter-slice dataflow model."""
+    """Compulsory HBM bytes of one launch of each group kernel (B positions, M modes,
+    n*n = NN pixels, c = 8 B complex64, r = 4 B float32)."""
     c, r = 8, 4
-    per_pos = {
-        # read psi_i (M planes), read O_i cutout, illumination read-modify-write, write psi_{i+1}
-        'forward_slice': 2 * M * c * NN + c * NN + 2 * r * NN,
-        # read chi, psi_i, O_i; write back-propagated chi; d_obj scatter read-modify-write
-        'lsqml_backward': 3 * M * c * NN + c * NN + 2 * c * NN,
-        # shifted probes: write psi0, illumination; slice-0 step reads O_0, writes psi1
-        'shift_probes': 2 * M * c * NN + c * NN + 2 * r * NN,
-        # exit wave: read psi, O, write spectrum, intensity r-m-w
-        'farfield_fwd': 2 * M * c * NN + c * NN + 2 * r * NN,
-        # modulus constraint: read spectrum, intensity, pattern; write chi
-        'farfield_update': 2 * M * c * NN + r * NN + r * NN,
+    plane = c * NN
+    per_launch = {
+        # read psi_i (M planes) and the O_i cutout, write psi_{i+1}; one propagator
+        'forward_slice': B * (2 * M * plane + plane) + plane,
+        # read chi, psi_i, O_i cutout; write back-propagated chi; one propagator
+        'lsqml_backward': B * (3 * M * plane + plane) + plane,
+        # read chi, psi_0, O_0 cutout; write the per-position probe spectrum
+        'lsqml_backward_slice0': B * (3 * M * plane + plane),
+        # write psi_0, read O_0 cutout, write psi_1; probe spectra and one propagator
+        'shift_probes': B * (2 * M * plane + plane) + M * plane + plane,
+        # read psi_{S-1}, O cutout; write spectrum + intensity; read them back with the
+        # pattern; write chi
+        'farfield': B * (2 * M * plane + plane + 2 * r * NN + 2 * M * plane + r * NN + r * NN),
+        # gather chi_i and psi_i of every slot; read-modify-write of the covered object
+        'lsqml_object_update': B * 2 * M * plane + 2 * c * min(HW, B * NN),
     }
-    return B * per_pos[name]
+    return per_launch[name]
 
 
 def lsqml_bytes_per_pattern_iter(S, M, NN, HW, B):
@@ -60,10 +64,12 @@ def time_kernels(ws, groups, torch, n_groups):
     HIP events recorded on the stream the kernels are launched on (torch's current
     stream)."""
     from phaser_amd import _lib
-    from phaser_amd.device import stream_handle
     durations = {}
 
-    def timed(name, fn, *args):
+    def timed(fn, *args):
+        name = fn[4:]
+        if fn == "phx_lsqml_backward" and args[1] == 0:
+            name += "_slice0"
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -71,19 +77,13 @@ def time_kernels(ws, groups, torch, n_groups):
         e1.record()
         durations.setdefault(name, []).append((e0, e1))
 
-    for g in range(n_groups):
-        d, st = ws.desc_refs[g], stream_handle()
-        _lib.call("phx_group_begin", d, st)
-        _lib.call("phx_probe_spectra", d, st)
-        timed('shift_probes', "phx_shift_probes", d, 1, int(ws.S > 1), st)
-        for i in range(1, ws.S - 1):
-            timed('forward_slice', "phx_forward_slice", d, i, 1, st)
-        timed('farfield', "phx_farfield", d, int(ws.S > 1), 0, st)
-        _lib.call("phx_group_obj_mag", d, st)
-        for i in range(ws.S - 1, -1, -1):
-            timed('lsqml_backward' if i > 0 else 'lsqml_backward_slice0', "phx_lsqml_backward", d, i, 1, 1, st)
-            timed('lsqml_object_apply', "phx_lsqml_object_apply", d, i, st)
-        timed('lsqml_probe_apply', "phx_lsqml_probe_apply", d, st)
+    ws.call = timed   # the engine's own launch sequence, each kernel bracketed by events
+    try:
+        for g in range(n_groups):
+            ws.launch_lsqml_group(g, update_object=True, update_probe=True, update_position=False,
+                                  calc_error=False)
+    finally:
+        ws.call = _lib.call
     torch.cuda.synchronize()
     out = {}
     for k, evs in durations.items():
@@ -122,7 +122,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-groups", type=int, default=1)
+    ap.add_argument("--cpu-groups", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
@@ -189,8 +189,9 @@ def main():
     # kernel durations (HIP events on the launch stream), one extra non-graph iteration
     ws.set_groups(list(groups.iter(state.scan, 999, True)))
     kt = time_kernels(ws, groups.iter, torch, G)
-    dom = max((k for k in kt if k in ('forward_slice', 'lsqml_backward', 'shift_probes')),
-              key=lambda k: kt[k]['total_ms'])
+    modeled = ('forward_slice', 'lsqml_backward', 'lsqml_backward_slice0', 'shift_probes', 'farfield',
+               'lsqml_object_update')
+    dom = max((k for k in kt if k in modeled), key=lambda k: kt[k]['total_ms'])
     NN, M, S, HW = ws.NN, ws.M, ws.S, ws.H * ws.W
     alg = kernel_bytes(dom, B, M, S, NN, HW)
     achieved = alg / (kt[dom]['avg_us'] * 1e-6) / 1e9
@@ -202,7 +203,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (numpy oracle, 1 thread)")
         cgroups = list(GroupManager(state0.scan, 64, False, seed=0).iter(state0.scan))
-        rate, npat, dt = cpu_baseline(state0, pats, cgroups, args.cpu_groups)
+        # the CPU sample starts from the GPU run's current (presolved, iterated) state
+        rate, npat, dt = cpu_baseline(state.to_numpy(), pats, cgroups, args.cpu_groups)
         cpu = {"value": rate, "unit": "patterns·iter/s", "cores": 1, "kind": "port",
                "sample": f"{npat} patterns x 1 LSQ-ML group step ({args.cpu_groups} group(s) of this workload) "
                          f"with the FP32 numpy oracle on the host, {dt:.1f} s; host has {os.cpu_count()} cores"}

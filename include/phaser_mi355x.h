@@ -121,7 +121,8 @@ typedef struct phx_group_desc {
 /* Doubles needed by phx_group_desc.red. */
 int64_t phx_red_doubles(int32_t n_slices, int32_t bmax, int32_t n_modes);
 
-/* Zero the per-group accumulators (gpm, intensity, red, errors). */
+/* Zero the per-group accumulators (gom, intensity, red, errors).  gpm, used only when a
+ * forward kernel is called with illum != 0, is the caller's to clear. */
 int phx_group_begin(const phx_group_desc* d, hipStream_t stream);
 
 /* probe_k[m] = FFT(probe[m]) (scrambled, unnormalised).  cutout_group's fft2(probes)
@@ -148,13 +149,20 @@ int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t
 int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t stream);
 
 /* LSQ-ML backward step at slice i (solvers.py:242-271).  Reads chi from chi_a (even
- * steps from the exit) or chi_b; when update_object, accumulates d_obj and the FP64
- * alpha_O sums; for i > 0 writes the back-propagated chi to the other buffer; for i == 0
- * and update_probe, accumulates d_probe and the FP64 alpha_P sums. */
+ * steps from the exit) or chi_b; when update_object, accumulates the FP64 alpha_O sums;
+ * for i > 0 writes the back-propagated chi to the other buffer; for i == 0 and
+ * update_probe, writes each slot's fft2(dP) conj(filt) to farfield and the FP64 alpha_P
+ * sums. */
 int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
                        hipStream_t stream);
-/* O_i += beta_O (sum_b alpha_O,b) d_obj/(gpm_i + reg) * gpm_i/(gpm_i + 1e-16); d_obj = 0. */
-int phx_lsqml_object_apply(const phx_group_desc* d, int slice, hipStream_t stream);
+/* LSQ-ML object update of slice i (solvers.py:213-215, 248-257), run after
+ * phx_lsqml_backward(i) and before the step of slice i-1:
+ * O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16) with
+ * dO = sum_{b,m} scatter(chi_i conj(psi_i)) and gpm = sum_{b,m} scatter(|psi_i|^2), both
+ * gathered per 32x32 object tile in position order (no atomics, deterministic). */
+int phx_lsqml_object_update(const phx_group_desc* d, int slice, hipStream_t stream);
+/* d_probe[m] = sum_b farfield[b][m] (position order): the probe-update spectrum. */
+int phx_probe_spectrum_reduce(const phx_group_desc* d, hipStream_t stream);
 /* P += beta_P (sum_b alpha_P) ifft2(d_probe)/(gom + reg) * gom/(gom + 1e-16); d_probe = 0. */
 int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream);
 

@@ -56,7 +56,8 @@ SIGNATURES = {
     "phx_forward_slice": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_farfield": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_lsqml_backward": (c_int, [_P, c_int, c_int, c_int, c_vp]),
-    "phx_lsqml_object_apply": (c_int, [_P, c_int, c_vp]),
+    "phx_lsqml_object_update": (c_int, [_P, c_int, c_vp]),
+    "phx_probe_spectrum_reduce": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply": (c_int, [_P, c_vp]),
     "phx_epie_backward": (c_int, [_P, c_int, c_int, c_int, c_vp]),
     "phx_epie_object_apply": (c_int, [_P, c_int, c_vp]),

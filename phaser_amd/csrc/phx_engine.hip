@@ -100,23 +100,20 @@ __global__ void __launch_bounds__(T) probe_spectra_k(phx_group_desc d) {
              [&](int py, int px, float2& v) { Pk[py * N + px] = v; });
 }
 
-// gom = sum_b |prod_s O_s|^2 over the group's cutouts (solvers.py:208).
+// gom += |prod_s O_s|^2 of slot b's cutout (solvers.py:208); grid (n*n/256, bmax), gom
+// zeroed by phx_group_begin.
 __global__ void __launch_bounds__(256) group_obj_mag_k(phx_group_desc d) {
   const int n = d.n, NN = n * n;
   const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= NN) return;
+  const int b = blockIdx.y;
+  if (e >= NN || !slot_active(d, b)) return;
   const int y = e / n, x = e % n;
-  const int gs = *d.group_size;
   const float2* obj = (const float2*)d.object;
   const size_t HW = (size_t)d.obj_h * d.obj_w;
-  float acc = 0.f;
-  for (int b = 0; b < gs; ++b) {
-    const Cut c = cut_of(d, d.group_pos[b]);
-    float2 prod = obj_at(obj, c, y, x);
-    for (int s = 1; s < d.n_slices; ++s) prod = cmul(prod, obj_at(obj + s * HW, c, y, x));
-    acc += abs2f(prod);
-  }
-  d.gom[e] = acc;
+  const Cut c = cut_of(d, d.group_pos[b]);
+  float2 prod = obj_at(obj, c, y, x);
+  for (int s = 1; s < d.n_slices; ++s) prod = cmul(prod, obj_at(obj + s * HW, c, y, x));
+  atomic_add_f32(&d.gom[e], abs2f(prod));
 }
 
 // psi0 = ifft2(fft2(P) * filt_b), illumination, norm, optional slice-0 step.
@@ -309,10 +306,11 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, i
   float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
   const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
   const float2* obj_i = (const float2*)d.object + i * HW;
-  float2* dO = (float2*)d.d_obj;
   const bool probe_step = (i == 0) && upd_probe;
   double numO = 0.0, denO = 0.0, numP = 0.0, denP = 0.0;
 
+  // delta_O = chi conj(psi_i) is not scattered here: lsqml_object_update_k gathers it
+  // (and the illumination sum |psi_i|^2) per object tile from chi and psi_i.
   auto src = [&](int y, int x) -> float2 {
     const float2 cv = chi[y * N + x];
     const bool inb = c.inb(y, x);
@@ -323,7 +321,6 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, i
       const float2 t = cmul(dOv, ps);
       numO += (double)(cv.x * t.x + cv.y * t.y);
       denO += (double)abs2f(t);
-      if (inb) atomic_add_c64(&dO[c.idx(y, x)], dOv);
     }
     const float2 dP = cmulc(cv, o);  // chi * conj(O)
     if (probe_step) {
@@ -339,10 +336,12 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, i
     pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
                 [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); });
   } else if (probe_step) {
+    // per-plane fft2(dP) conj(filt_b) into the (free by now) farfield buffer; the sum
+    // over positions is probe_spectrum_reduce_k
     const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
-    float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+    float2* Fp = (float2*)d.farfield + ((size_t)b * M + m) * NN;
     pl.forward(src, [&](int py, int px, float2& v) {
-      atomic_add_c64(&dPk[py * N + px], cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px])));
+      Fp[py * N + px] = cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]));
     });
   } else {
     for (int e = threadIdx.x; e < NN; e += T) (void)src(e / N, e % N);
@@ -367,12 +366,19 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, i
   }
 }
 
-// O_i += beta (sum_b alpha_b) d_obj/(gpm+reg) * gpm/(gpm+eps); d_obj = 0 (solvers.py:251-257).
-__global__ void __launch_bounds__(256) lsqml_object_apply_k(phx_group_desc d, int i) {
+// LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
+//   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
+//   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
+// One workgroup owns a 16x16 object tile (one pixel per thread) and gathers both sums
+// from the group's chi_i and psi_i planes, positions in group order and modes in order:
+// no atomics, and the result does not depend on scheduling.  Pixels no cutout covers are
+// left untouched (their reference update is exactly zero).
+constexpr int UPD_TILE = 16;
+__global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int i) {
   __shared__ float sA;
   const RedLayout L = red_layout(d.n_slices, d.bmax, d.n_modes);
+  const int gs = *d.group_size;
   if (threadIdx.x < 64) {
-    const int gs = *d.group_size;
     double s = 0.0;
     for (int b = threadIdx.x; b < gs; b += 64) s += d.red[L.num_obj + i * d.bmax + b];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -383,22 +389,64 @@ __global__ void __launch_bounds__(256) lsqml_object_apply_k(phx_group_desc d, in
   }
   __syncthreads();
   const float A = sA, reg = d.params[P_REG_OBJ];
-  const size_t HW = (size_t)d.obj_h * d.obj_w;
-  float2* obj_i = (float2*)d.object + i * HW;
-  const float* gpm_i = d.gpm + i * HW;
-  float2* dO = (float2*)d.d_obj;
-  bool bad = false;
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < HW; e += (size_t)gridDim.x * 256) {
-    const float g = gpm_i[e];
-    const float2 q = cscale(dO[e], 1.0f / (g + reg));
-    const float2 u = cscale(cscale(cscale(q, A), g), 1.0f / (g + 1e-16f));
-    float2 o = obj_i[e];
-    o = cadd(o, u);
-    obj_i[e] = o;
-    dO[e] = make_float2(0.f, 0.f);
-    bad |= !(isfinite(o.x) && isfinite(o.y));
+  const int n = d.n, NN = n * n, M = d.n_modes, H = d.obj_h, W = d.obj_w;
+  const int ty0 = blockIdx.y * UPD_TILE, tx0 = blockIdx.x * UPD_TILE;
+  const int y = ty0 + threadIdx.x / UPD_TILE, x = tx0 + threadIdx.x % UPD_TILE;
+  const float2* chi = chi_in_buf(d, i);
+  const float2* psi = (const float2*)d.psi;
+  const size_t mstride_psi = (size_t)NN;  // consecutive modes of one slot are adjacent planes
+  float2 dO = make_float2(0.f, 0.f);
+  float g = 0.f;
+  for (int b = 0; b < gs; ++b) {
+    const int p = d.group_pos[b];
+    const int y0 = d.start[2 * p], x0 = d.start[2 * p + 1];
+    if (y0 >= ty0 + UPD_TILE || y0 + n <= ty0 || x0 >= tx0 + UPD_TILE || x0 + n <= tx0) continue;
+    const int ly = y - y0, lx = x - x0;
+    if (ly < 0 || ly >= n || lx < 0 || lx >= n) continue;
+    const size_t off = (size_t)ly * n + lx;
+    const float2* cb = chi + (size_t)b * M * NN + off;
+    const float2* pb = psi + psi_off(d, i, b, 0) + off;
+    int m = 0;
+    for (; m + 4 <= M; m += 4) {
+      const float2 c0 = cb[(m + 0) * mstride_psi], c1 = cb[(m + 1) * mstride_psi];
+      const float2 c2 = cb[(m + 2) * mstride_psi], c3 = cb[(m + 3) * mstride_psi];
+      const float2 p0 = pb[(m + 0) * mstride_psi], p1 = pb[(m + 1) * mstride_psi];
+      const float2 p2 = pb[(m + 2) * mstride_psi], p3 = pb[(m + 3) * mstride_psi];
+      dO = cadd(dO, cmulc(c0, p0));
+      dO = cadd(dO, cmulc(c1, p1));
+      dO = cadd(dO, cmulc(c2, p2));
+      dO = cadd(dO, cmulc(c3, p3));
+      g += abs2f(p0);
+      g += abs2f(p1);
+      g += abs2f(p2);
+      g += abs2f(p3);
+    }
+    for (; m < M; ++m) {
+      const float2 cv = cb[m * mstride_psi], ps = pb[m * mstride_psi];
+      dO = cadd(dO, cmulc(cv, ps));
+      g += abs2f(ps);
+    }
   }
-  if (bad) atomicMin(d.finite_flag, d.group_index);
+  if (y >= H || x >= W || (g == 0.f && dO.x == 0.f && dO.y == 0.f)) return;
+  float2* obj_i = (float2*)d.object + (size_t)i * H * W;
+  const float2 q = cscale(dO, 1.0f / (g + reg));
+  const float2 u = cscale(cscale(cscale(q, A), g), 1.0f / (g + 1e-16f));
+  const float2 o = cadd(obj_i[(size_t)y * W + x], u);
+  obj_i[(size_t)y * W + x] = o;
+  if (!(isfinite(o.x) && isfinite(o.y))) atomicMin(d.finite_flag, d.group_index);
+}
+
+// d_probe[m] = sum_b farfield[b][m]: the per-position probe-update spectra written by
+// the slice-0 backward step, summed over the group in position order.
+__global__ void __launch_bounds__(256) probe_spectrum_reduce_k(phx_group_desc d) {
+  const int NN = d.n * d.n, M = d.n_modes, m = blockIdx.y;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= NN) return;
+  const int gs = *d.group_size;
+  const float2* F = (const float2*)d.farfield;
+  float2 s = make_float2(0.f, 0.f);
+  for (int b = 0; b < gs; ++b) s = cadd(s, F[((size_t)b * M + m) * NN + e]);
+  ((float2*)d.d_probe)[(size_t)m * NN + e] = s;
 }
 
 // P += beta_P (sum_b alpha_P) ifft2(sum_b fft2(dP) conj(filt))/(gom+reg) * gom/(gom+eps).
@@ -783,14 +831,14 @@ int64_t phx_red_doubles(int32_t n_slices, int32_t bmax, int32_t n_modes) {
 int phx_group_begin(const phx_group_desc* d, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (!d->gpm || !d->intensity || !d->errors) return PHX_EINVAL;
-  const int64_t na = (int64_t)d->n_slices * d->obj_h * d->obj_w;
+  if (!d->gom || !d->intensity || !d->errors) return PHX_EINVAL;
+  const int64_t na = (int64_t)d->n * d->n;
   const int64_t nb = (int64_t)d->bmax * d->n * d->n;
   const int64_t nc = red_layout(d->n_slices, d->bmax, d->n_modes).total;
   const int64_t ne = d->bmax;
   int64_t blocks = (na + nb + nc + ne + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(zero4_k, dim3((unsigned)blocks), dim3(256), 0, stream, d->gpm, na, d->intensity, nb, d->red, nc,
+  hipLaunchKernelGGL(zero4_k, dim3((unsigned)blocks), dim3(256), 0, stream, d->gom, na, d->intensity, nb, d->red, nc,
                      d->errors, ne);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
@@ -812,7 +860,7 @@ int phx_group_obj_mag(const phx_group_desc* d, hipStream_t stream) {
   if (r) return r;
   if (!d->gom) return PHX_EINVAL;
   const int NN = d->n * d->n;
-  hipLaunchKernelGGL(group_obj_mag_k, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, stream, *d);
+  hipLaunchKernelGGL(group_obj_mag_k, dim3((unsigned)((NN + 255) / 256), (unsigned)d->bmax), dim3(256), 0, stream, *d);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }
@@ -875,14 +923,22 @@ int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, in
   return PHX_OK;
 }
 
-int phx_lsqml_object_apply(const phx_group_desc* d, int slice, hipStream_t stream) {
+int phx_lsqml_object_update(const phx_group_desc* d, int slice, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (slice < 0 || slice >= d->n_slices || !d->d_obj || !d->gpm || !d->finite_flag) return PHX_EINVAL;
-  const int64_t HW = (int64_t)d->obj_h * d->obj_w;
-  int64_t blocks = (HW + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(lsqml_object_apply_k, dim3((unsigned)blocks), dim3(256), 0, stream, *d, slice);
+  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || !d->finite_flag) return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->obj_w + UPD_TILE - 1) / UPD_TILE), (unsigned)((d->obj_h + UPD_TILE - 1) / UPD_TILE));
+  hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, slice);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_probe_spectrum_reduce(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->farfield || !d->d_probe) return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->n * d->n + 255) / 256), (unsigned)d->n_modes);
+  hipLaunchKernelGGL(probe_spectrum_reduce_k, grid, dim3(256), 0, stream, *d);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

@@ -130,6 +130,9 @@ class GroupWorkspace:
         self.probe = probe
         self.descs = [self._make_desc(g) for g in range(self.G)]
         self.desc_refs = [ctypes.byref(d) for d in self.descs]
+        # every launch of the group sequences goes through this callable (name, *args);
+        # a profiler may wrap it to bracket each kernel with events
+        self.call = _lib.call
 
     def _make_desc(self, g):
         d = _lib.GroupDesc()
@@ -198,7 +201,7 @@ class GroupWorkspace:
     # ------------------------------------------------------------------ launches
     def launch_dry_run(self, g):
         d, st = self.desc_refs[g], stream_handle()
-        c = _lib.call
+        c = self.call
         c("phx_group_begin", d, st)
         c("phx_probe_spectra", d, st)
         c("phx_shift_probes", d, 0, int(self.S > 1), st)
@@ -208,7 +211,7 @@ class GroupWorkspace:
 
     def launch_forward(self, g, calc_error, illum=True):
         d, st = self.desc_refs[g], stream_handle()
-        c = _lib.call
+        c = self.call
         c("phx_group_begin", d, st)
         c("phx_probe_spectra", d, st)
         c("phx_shift_probes", d, int(illum), int(self.S > 1), st)
@@ -219,8 +222,9 @@ class GroupWorkspace:
     def launch_lsqml_group(self, g, *, update_object, update_probe, update_position, calc_error):
         """lsqml_run (solvers.py:171-290) for group slot g."""
         d, st = self.desc_refs[g], stream_handle()
-        c = _lib.call
-        self.launch_forward(g, calc_error, illum=True)
+        c = self.call
+        # illumination sums are gathered by the object update, not scattered here
+        self.launch_forward(g, calc_error, illum=False)
         if update_probe:
             c("phx_group_obj_mag", d, st)
         for i in range(self.S - 1, -1, -1):
@@ -230,19 +234,20 @@ class GroupWorkspace:
                 # final chi (solvers.py:275-288), so it runs before slice 0 is updated
                 self._launch_position(d, st)
             if update_object:
-                c("phx_lsqml_object_apply", d, i, st)
+                c("phx_lsqml_object_update", d, i, st)
         if update_probe:
+            c("phx_probe_spectrum_reduce", d, st)
             c("phx_lsqml_probe_apply", d, st)
 
     def _launch_position(self, d, st):
-        _lib.call("phx_position_gradient", d, 0, st)
-        _lib.call("phx_position_gradient", d, 1, st)
-        _lib.call("phx_position_finish", d, _ptr(self.pos_update), st)
+        self.call("phx_position_gradient", d, 0, st)
+        self.call("phx_position_gradient", d, 1, st)
+        self.call("phx_position_finish", d, _ptr(self.pos_update), st)
 
     def launch_epie_group(self, g, *, update_object, update_probe, update_position, calc_error):
         """epie_run (solvers.py:415-504) for group slot g."""
         d, st = self.desc_refs[g], stream_handle()
-        c = _lib.call
+        c = self.call
         self.launch_forward(g, calc_error, illum=False)
         for i in range(self.S - 1, -1, -1):
             c("phx_epie_backward", d, i, int(update_object), int(update_probe), st)
