@@ -71,8 +71,8 @@ int phx_scale_c64(void* ptr, int64_t count, float scale, hipStream_t stream);
  *
  * Frequency-domain arrays (propagators, patterns, mask, k-grids, probe spectra) are kept
  * in the FFT kernels' "scrambled" order: along each axis of an n-point transform,
- * position p holds frequency (p / R2) + R1 * (p % R2) where n = R1 * R2 (R1 = 16, 8, 8, 4;
- * R2 = 8, 8, 4, 4 for n = 128, 64, 32, 16).  Real-space arrays are natural order.
+ * position p holds frequency (p / 4) + (n / 4) * bitrev2(p % 4), bitrev2 = {0, 2, 1, 3}
+ * (n = 16, 32, 64, 128).  Real-space arrays are natural order.
  * --------------------------------------------------------------------------------- */
 typedef struct phx_group_desc {
   int32_t n;          /* plane side, 16..128, power of two */

@@ -73,9 +73,10 @@ __global__ void fill_u32_k(uint32_t* p, uint32_t v, int64_t n) {
 template <int N, int T>
 __global__ void __launch_bounds__(T) fft2_conv_k(const float2* __restrict__ in, float2* __restrict__ out,
                                                  const float2* __restrict__ twg, int inverse) {
-  __shared__ float2 lds[Plane<N, T>::LDS_FLOAT2 + N];
-  Plane<N, T> pl{lds, lds + Plane<N, T>::LDS_FLOAT2};
-  load_twiddles<N, T>(lds + Plane<N, T>::LDS_FLOAT2, twg);
+  static_assert(T == Plane<N>::T, "plane kernels run 4n threads");
+  __shared__ float2 lds[Plane<N>::LDS_FLOAT2 + N];
+  Plane<N> pl(lds, lds + Plane<N>::LDS_FLOAT2);
+  load_twiddles<N>(lds + Plane<N>::LDS_FLOAT2, twg);
   __syncthreads();
   const float2* a = in + (size_t)blockIdx.x * N * N;
   float2* b = out + (size_t)blockIdx.x * N * N;
@@ -199,8 +200,8 @@ int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int invers
   if (in == out) return PHX_EINVAL;
   if (batch == 0) return PHX_OK;
   switch (n) {
-    case 16: return launch_fft2<16, 256>(in, out, batch, inverse, twiddle, stream);
-    case 32: return launch_fft2<32, 256>(in, out, batch, inverse, twiddle, stream);
+    case 16: return launch_fft2<16, 64>(in, out, batch, inverse, twiddle, stream);
+    case 32: return launch_fft2<32, 128>(in, out, batch, inverse, twiddle, stream);
     case 64: return launch_fft2<64, 256>(in, out, batch, inverse, twiddle, stream);
     case 128: return launch_fft2<128, 512>(in, out, batch, inverse, twiddle, stream);
     default: return PHX_EUNSUPPORTED;

@@ -47,8 +47,8 @@ enum FarMode { FM_MODULUS = 0, FM_GRAD_AMPLITUDE = 1, FM_GRAD_POISSON = 2 };
 struct Cut {
   int y0, x0, H, W;
   __device__ __forceinline__ bool inb(int y, int x) const {
-    const int gy = y0 + y, gx = x0 + x;
-    return gy >= 0 && gy < H && gx >= 0 && gx < W;
+    // unsigned compares joined without short-circuit: no branches in unrolled loops
+    return ((unsigned)(y0 + y) < (unsigned)H) & ((unsigned)(x0 + x) < (unsigned)W);
   }
   __device__ __forceinline__ int idx(int y, int x) const { return (y0 + y) * W + (x0 + x); }
 };
@@ -58,23 +58,30 @@ __device__ __forceinline__ Cut cut_of(const phx_group_desc& d, int p) {
 }
 
 // Object value under the cutout; out of bounds gives 1 (cupy default, object.py:509).
+// Branch-free: the load goes to the clamped (always valid) address and the result is
+// selected, so an unrolled loop over a row stays one basic block the scheduler can pace.
 __device__ __forceinline__ float2 obj_at(const float2* o, const Cut& c, int y, int x) {
-  return c.inb(y, x) ? o[c.idx(y, x)] : make_float2(1.f, 0.f);
+  const int gy = min(max(c.y0 + y, 0), c.H - 1), gx = min(max(c.x0 + x, 0), c.W - 1);
+  const float2 v = o[gy * c.W + gx];
+  return c.inb(y, x) ? v : make_float2(1.f, 0.f);
 }
 
-// exp(-2 pi i (sx * kx + sy * ky)) in float32 (optics.py:151-170)
+// exp(-2 pi i (sx * kx + sy * ky)) in float32 (optics.py:151-170).  The phase is taken
+// in half-turns (sincospi: exact integer range reduction, no large-argument path); it
+// differs from exp of the float32-rounded product -2 pi t by that rounding only (~1e-7).
 __device__ __forceinline__ float2 shift_filter(float sy, float sx, float ky, float kx) {
-  const float th = -TWO_PI * (sx * kx + sy * ky);
+  const float t = sx * kx + sy * ky;
   float sn, cs;
-  sincosf(th, &sn, &cs);
+  sincospif(-2.0f * t, &sn, &cs);
   return make_float2(cs, sn);
 }
 
-#define PHX_PLANE_SETUP(N, T)                                                 \
-  __shared__ float2 lds_[Plane<N, T>::LDS_FLOAT2 + N];                        \
-  __shared__ double red_lds_[T / 64];                                         \
-  Plane<N, T> pl{lds_, lds_ + Plane<N, T>::LDS_FLOAT2};                       \
-  load_twiddles<N, T>(lds_ + Plane<N, T>::LDS_FLOAT2, (const float2*)d.twiddle); \
+#define PHX_PLANE_SETUP(N, T)                                                    \
+  static_assert(T == Plane<N>::T, "plane kernels run 4n threads");               \
+  __shared__ float2 lds_[Plane<N>::LDS_FLOAT2 + N];                              \
+  __shared__ double red_lds_[T / 64];                                            \
+  Plane<N> pl(lds_, lds_ + Plane<N>::LDS_FLOAT2);                                \
+  load_twiddles<N>(lds_ + Plane<N>::LDS_FLOAT2, (const float2*)d.twiddle);       \
   __syncthreads();
 
 __device__ __forceinline__ bool slot_active(const phx_group_desc& d, int b) { return b < *d.group_size; }
@@ -134,21 +141,26 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum,
   float* gpm0 = d.gpm;
   double nrm = 0.0;
 
-  pl.cols_iA([&](int py, int px) { return cmul(Pk[py * N + px], shift_filter(sy, sx, d.ky_s[py], d.kx_s[px])); });
-  __syncthreads();
-  pl.cols_iB();
-  __syncthreads();
-  pl.rows_iA();
-  __syncthreads();
-  pl.rows_iB([&](int y, int x, float2 v) {
-    v = cscale(v, inv_nn);
-    psi0[y * N + x] = v;
+  constexpr int E = Plane<N>::E;
+  float2 x[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int py = 4 * k + pl.q, px = pl.rc;
+    x[k] = cmul(Pk[py * N + px], shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]));
+  }
+  pl.inverse_regs(x);
+  // x now holds psi0 (unscaled) at row rc, columns 4j + q
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int yy = pl.rc, xx = 4 * j + pl.q;
+    const float2 v = cscale(x[j], inv_nn);
+    psi0[yy * N + xx] = v;
     const float a = abs2f(v);
     nrm += a;
-    const bool inb = c.inb(y, x);
-    if (illum && inb) atomic_add_f32(&gpm0[c.idx(y, x)], a);
-    if (propagate) pl.at(y, x) = cmul(v, inb ? obj0[c.idx(y, x)] : make_float2(1.f, 0.f));
-  });
+    const bool inb = c.inb(yy, xx);
+    if (illum && inb) atomic_add_f32(&gpm0[c.idx(yy, xx)], a);
+    x[j] = cmul(v, inb ? obj0[c.idx(yy, xx)] : make_float2(1.f, 0.f));
+  }
   const double tot = block_sum_f64(nrm, red_lds_);
   if (threadIdx.x == 0) {
     RedLayout L = red_layout(d.n_slices, bmax, M);
@@ -157,9 +169,8 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum,
   if (!propagate) return;
   const float2* prop = (const float2*)d.props;
   float2* psi1 = (float2*)d.psi + psi_off(d, 1, b, m);
-  pl.convolve([&](int y, int x) { return pl.at(y, x); },
-              [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); },
-              [&](int y, int x, float2 v) { psi1[y * N + x] = cscale(v, inv_nn); });
+  pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); });
+  pl.store_rows([&](int yy, int xx, float2 v) { psi1[yy * N + xx] = cscale(v, inv_nn); }, x);
 }
 
 template <int N, int T>
@@ -290,52 +301,60 @@ __device__ __forceinline__ float2* chi_out_buf(const phx_group_desc& d, int i) {
   return (float2*)(((d.n_slices - 1 - i) % 2 == 0) ? d.chi_b : d.chi_a);
 }
 
-// LSQ-ML backward step at slice i (solvers.py:242-271).
-template <int N, int T>
-__global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, int upd_obj, int upd_probe) {
+// LSQ-ML backward step at slice i (solvers.py:242-271).  The variant is compile-time:
+// BK_PROPAGATE (i > 0: back-propagate chi through slice i-1's propagator),
+// BK_PROBE (i == 0 with a probe update: per-plane probe spectrum), BK_NONE (i == 0, no
+// probe update: step-size sums only); UPD_OBJ adds the object step-size sums.  The
+// row source is branch-free, so the unrolled load phase is one basic block.
+enum BackKind { BK_PROPAGATE = 0, BK_PROBE = 1, BK_NONE = 2 };
+template <int N, int T, int KIND, bool UPD_OBJ>
+__global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
   const int m = blockIdx.x, b = blockIdx.y;
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
   constexpr int NN = N * N;
   constexpr float inv_nn = 1.0f / (float)NN;
+  constexpr bool PROBE = KIND == BK_PROBE;
   const int M = d.n_modes, bmax = d.bmax;
   const int p = d.group_pos[b];
   const Cut c = cut_of(d, p);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float2* chi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
-  float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
   const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
   const float2* obj_i = (const float2*)d.object + i * HW;
-  const bool probe_step = (i == 0) && upd_probe;
-  double numO = 0.0, denO = 0.0, numP = 0.0, denP = 0.0;
+  // per-lane partial sums over its n/4 points in FP32, block and group sums in FP64
+  float numO = 0.f, denO = 0.f, numP = 0.f, denP = 0.f;
 
   // delta_O = chi conj(psi_i) is not scattered here: lsqml_object_update_k gathers it
   // (and the illumination sum |psi_i|^2) per object tile from chi and psi_i.
   auto src = [&](int y, int x) -> float2 {
     const float2 cv = chi[y * N + x];
-    const bool inb = c.inb(y, x);
-    const float2 o = inb ? obj_i[c.idx(y, x)] : make_float2(1.f, 0.f);
-    if (upd_obj) {
+    const float2 o = obj_at(obj_i, c, y, x);
+    if (UPD_OBJ) {
       const float2 ps = psi_i[y * N + x];
-      const float2 dOv = cmulc(cv, ps);  // chi * conj(psi)
-      const float2 t = cmul(dOv, ps);
-      numO += (double)(cv.x * t.x + cv.y * t.y);
-      denO += (double)abs2f(t);
+      const float2 t = cmul(cmulc(cv, ps), ps);  // (chi conj(psi)) psi
+      numO += cv.x * t.x + cv.y * t.y;
+      denO += abs2f(t);
+      // keeps the sums in the load phase: sunk to their use after the transform they
+      // would hold every point's inputs live and spill at n = 128
+      __asm__ volatile("" : "+v"(numO), "+v"(denO));
     }
     const float2 dP = cmulc(cv, o);  // chi * conj(O)
-    if (probe_step) {
+    if (PROBE) {
       const float2 t2 = cmul(dP, o);
-      numP += (double)(cv.x * t2.x + cv.y * t2.y);
-      denP += (double)abs2f(t2);
+      numP += cv.x * t2.x + cv.y * t2.y;
+      denP += abs2f(t2);
+      __asm__ volatile("" : "+v"(numP), "+v"(denP));
     }
     return dP;
   };
 
-  if (i > 0) {
+  if (KIND == BK_PROPAGATE) {
+    float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
     const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
     pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
                 [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); });
-  } else if (probe_step) {
+  } else if (PROBE) {
     // per-plane fft2(dP) conj(filt_b) into the (free by now) farfield buffer; the sum
     // over positions is probe_spectrum_reduce_k
     const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
@@ -348,7 +367,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, i
   }
 
   const RedLayout L = red_layout(d.n_slices, bmax, M);
-  if (upd_obj) {
+  if (UPD_OBJ) {
     const double a = block_sum_f64(numO, red_lds_);
     const double bb = block_sum_f64(denO, red_lds_);
     if (threadIdx.x == 0) {
@@ -356,7 +375,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i, i
       atomic_add_f64(&d.red[L.den_obj + i], bb);
     }
   }
-  if (probe_step) {
+  if (PROBE) {
     const double a = block_sum_f64(numP, red_lds_);
     const double bb = block_sum_f64(denP, red_lds_);
     if (threadIdx.x == 0) {
@@ -635,7 +654,7 @@ __global__ void __launch_bounds__(T) position_gradient_k(phx_group_desc d, int d
   const float2* chi = (const float2*)((d.n_slices % 2 == 1) ? d.chi_a : d.chi_b) + ((size_t)b * M + m) * NN;
   const float2* obj0 = (const float2*)d.object;
   const float scale = (float)(1.0 / ((double)NN * d.red[L.norm + b]));
-  double num = 0.0, den = 0.0;
+  float num = 0.f, den = 0.f;  // per-lane FP32 partials, FP64 block sums
   pl.convolve([&](int y, int x) { return psi0[y * N + x]; },
               [&](int py, int px, float2 v) {
                 const float k = dir == 0 ? d.ky_s[py] : d.kx_s[px];
@@ -645,8 +664,8 @@ __global__ void __launch_bounds__(T) position_gradient_k(phx_group_desc d, int d
               [&](int y, int x, float2 v) {
                 const float2 prod = cmul(cscale(v, scale), obj_at(obj0, c, y, x));
                 const float2 cv = chi[y * N + x];
-                num += (double)(cv.x * prod.x + cv.y * prod.y);
-                den += (double)abs2f(prod);
+                num += cv.x * prod.x + cv.y * prod.y;
+                den += abs2f(prod);
               });
   const double a = block_sum_f64(num, red_lds_);
   const double bb = block_sum_f64(den, red_lds_);
@@ -696,15 +715,15 @@ __global__ void __launch_bounds__(T) gradient_backward_k(phx_group_desc d, int i
     const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
     const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
     float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
-    double gy = 0.0, gx = 0.0;
+    float gy = 0.f, gx = 0.f;  // per-lane FP32 partials, FP64 block sums
     pl.forward(src, [&](int py, int px, float2& v) {
       const int e = py * N + px;
       const float2 f = shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]);
       if (want_probe) atomic_add_c64(&dPk[e], cmulc(v, f));
       if (want_pos) {
         const float2 t = cmulc(cmul(Pk[e], f), v);  // conj(v) * fft2(P) filt
-        gy += (double)(TWO_PI * d.ky_s[py] * t.y);
-        gx += (double)(TWO_PI * d.kx_s[px] * t.y);
+        gy += TWO_PI * d.ky_s[py] * t.y;
+        gx += TWO_PI * d.kx_s[px] * t.y;
       }
     });
     if (want_pos) {
@@ -813,14 +832,34 @@ int check_desc(const phx_group_desc* d) {
 
 #define PHX_DISPATCH(n, LAUNCH)       \
   switch (n) {                        \
-    case 16: LAUNCH(16, 256); break;  \
-    case 32: LAUNCH(32, 256); break;  \
+    case 16: LAUNCH(16, 64); break;   \
+    case 32: LAUNCH(32, 128); break;  \
     case 64: LAUNCH(64, 256); break;  \
     case 128: LAUNCH(128, 512); break;\
     default: return PHX_EUNSUPPORTED; \
   }
 
 }  // namespace
+
+template <int N, int T>
+static void launch_lsqml_backward(const phx_group_desc& d, int slice, int upd_obj, int upd_probe,
+                                  hipStream_t stream) {
+  const dim3 grid = PHX_PLANE_GRID(&d), block(T);
+#define K(KIND)                                                                                        \
+  do {                                                                                                 \
+    if (upd_obj)                                                                                       \
+      hipLaunchKernelGGL((lsqml_backward_k<N, T, KIND, true>), grid, block, 0, stream, d, slice);      \
+    else                                                                                               \
+      hipLaunchKernelGGL((lsqml_backward_k<N, T, KIND, false>), grid, block, 0, stream, d, slice);     \
+  } while (0)
+  if (slice > 0)
+    K(BK_PROPAGATE);
+  else if (upd_probe)
+    K(BK_PROBE);
+  else
+    K(BK_NONE);
+#undef K
+}
 
 extern "C" {
 
@@ -914,9 +953,7 @@ int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, in
   if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (update_object && !d->d_obj) ||
       (update_probe && !d->d_probe))
     return PHX_EINVAL;
-#define L(N, T)                                                                                            \
-  hipLaunchKernelGGL((lsqml_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, update_object, \
-                     update_probe)
+#define L(N, T) launch_lsqml_backward<N, T>(*d, slice, update_object, update_probe, stream)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();

@@ -1,63 +1,62 @@
-// LDS-resident batched 2-D complex FFT for n x n planes (n = 16, 32, 64, 128) on gfx950.
+// Batched 2-D complex FFT of n x n planes (n = 16, 32, 64, 128) for gfx950, one plane
+// per workgroup of 4n threads (n/16 wavefronts).
 //
-// One workgroup owns one plane.  The plane lives in LDS (n x (n+1) float2, padded row
-// stride), so a 128^2 complex64 plane takes 129 KiB of the 160 KiB LDS of a CU.
+// Layout.  Four lanes (a "quad") own one row (row phase) or one column (column phase) of
+// the plane; lane q of the quad holds the E = n/4 points with index 4 j + q, j < E.
+// A length-n transform along the quad is a four-step n = E * 4 decomposition:
+//   1. in-register DFT_E over j (radix-2, compile-time twiddles),
+//   2. twiddle W_n^(q k1) (table in LDS),
+//   3. DFT_4 across the quad in two radix-2 stages of DPP quad permutes (no LDS).
+// Lane q then holds X[k1 + E bitrev2(q)], k1 < E, which it stores at "scrambled"
+// position p = 4 k1 + q: position p holds frequency f(p) = p/4 + E bitrev2(p % 4).
+// The inverse runs the same steps backwards and returns natural order.
 //
-// Each 1-D length-n transform is a four-step decomposition n = R1 * R2 (R1, R2 <= 16)
-// done in two register passes with an LDS exchange:
-//   forward stage 1 : for each n2 < R2, DFT_R1 over x[R2*n1 + n2]; twiddle W_n^(n2*k1);
-//                     write back in place at R2*k1 + n2.
-//   forward stage 2 : for each k1 < R1, DFT_R2 over the R2 contiguous entries R2*k1 + j;
-//                     write back in place.  Position p = R2*k1 + k2 now holds X[k1 + R1*k2]
-//                     ("scrambled" order).
-//   inverse stage A : exact reverse of forward stage 2 (inverse DFT_R2, in place).
-//   inverse stage B : conj twiddle, inverse DFT_R1 over the stride-R2 entries; output is in
-//                     natural order, in place.
-// So a forward transform takes natural order to scrambled order and an inverse transform
-// takes scrambled order back to natural order, without any permutation pass.  Pointwise
-// work between them (propagators, noise-model updates, shift filters) is done in the
-// scrambled order with frequency-domain constants stored in that same order.
+// A 2-D transform is: rows in registers -> transpose through LDS -> columns in registers.
+// The LDS plane has row stride n + 4 float2, which makes both the row-phase writes
+// (s[r][4k + q]) and the column-phase reads (s[4j + q][c]) bank-conflict free.  A forward
+// transform, a pointwise multiply in the (scrambled) frequency domain and an inverse
+// transform therefore cost two LDS round trips in total.
 //
-// Transforms are unnormalised: forward uses exp(-2 pi i nk/n), inverse exp(+2 pi i nk/n).
-// The (i)fftshift of the centred real-space convention cancels in every product the
-// engines form (fft2 -> pointwise -> ifft2), so it is not applied here; callers scale by
-// 1/n^2 per forward+inverse pair (ortho normalisation).
-//
-// The first stage of the row pass can read straight from global memory (fused load)
-// and the last stage of the inverse row pass can write straight to global memory
-// (fused store).  Forward stage 2 and inverse stage A of the column pass touch the same
-// R2 entries, so a pointwise multiply between a forward and an inverse transform is done
-// in registers with no LDS round trip (cols_f2_mul_iA).
+// Transforms are unnormalised: forward uses exp(-2 pi i nk/n), inverse exp(+2 pi i nk/n),
+// so inverse(forward(a)) = n^2 a; callers scale by 1/n^2.  The (i)fftshift of the centred
+// real-space convention cancels in every fft2 -> pointwise -> ifft2 product the engines
+// form, so it is not applied here.
 #pragma once
 #include "phx_common.h"
 
 namespace phx {
 
-// exp(-2 pi i k / 16), k in [0, 16)
-__device__ __forceinline__ float2 w16(int k) {
-  constexpr float C8 = 0.92387953251128674f, S8 = 0.38268343236508978f, R2f = 0.70710678118654752f;
-  constexpr float c[16] = {1.f, C8, R2f, S8, 0.f, -S8, -R2f, -C8, -1.f, -C8, -R2f, -S8, 0.f, S8, R2f, C8};
-  constexpr float s[16] = {0.f, -S8, -R2f, -C8, -1.f, -C8, -R2f, -S8, 0.f, S8, R2f, C8, 1.f, C8, R2f, S8};
+// exp(-2 pi i k / 32), k in [0, 32)
+__device__ __forceinline__ float2 w32(int k) {
+  constexpr float C1 = 0.98078528040323044f, C2 = 0.92387953251128676f, C3 = 0.83146961230254524f,
+                  C4 = 0.70710678118654752f, C5 = 0.55557023301960222f, C6 = 0.38268343236508977f,
+                  C7 = 0.19509032201612826f;
+  constexpr float c[32] = {1.f, C1, C2, C3, C4, C5, C6, C7, 0.f, -C7, -C6, -C5, -C4, -C3, -C2, -C1,
+                           -1.f, -C1, -C2, -C3, -C4, -C5, -C6, -C7, 0.f, C7, C6, C5, C4, C3, C2, C1};
+  // -sin(2 pi k/32) = c[(k + 8) % 32] * -1 ... written out for clarity
+  constexpr float s[32] = {0.f, -C7, -C6, -C5, -C4, -C3, -C2, -C1, -1.f, -C1, -C2, -C3, -C4, -C5, -C6, -C7,
+                           0.f, C7, C6, C5, C4, C3, C2, C1, 1.f, C1, C2, C3, C4, C5, C6, C7};
   return make_float2(c[k], s[k]);
 }
 
-// v * exp(-2 pi i k / 16) with the trivial factors special-cased (k is a compile-time
-// constant after unrolling).
-__device__ __forceinline__ float2 twmul16(float2 v, int k) {
+// v * exp(-2 pi i k / 32) (k compile-time after unrolling; trivial factors special-cased)
+__device__ __forceinline__ float2 twmul32(float2 v, int k) {
   if (k == 0) return v;
-  if (k == 4) return make_float2(v.y, -v.x);
-  if (k == 8) return make_float2(-v.x, -v.y);
-  if (k == 12) return make_float2(-v.y, v.x);
-  return cmul(v, w16(k));
+  if (k == 8) return make_float2(v.y, -v.x);
+  if (k == 16) return make_float2(-v.x, -v.y);
+  if (k == 24) return make_float2(-v.y, v.x);
+  return cmul(v, w32(k));
 }
 
 template <int R> struct Log2;
+template <> struct Log2<1> { static constexpr int v = 0; };
 template <> struct Log2<2> { static constexpr int v = 1; };
 template <> struct Log2<4> { static constexpr int v = 2; };
 template <> struct Log2<8> { static constexpr int v = 3; };
 template <> struct Log2<16> { static constexpr int v = 4; };
+template <> struct Log2<32> { static constexpr int v = 5; };
 
-// In-register length-R DFT (R = 2, 4, 8, 16), natural order in and out.
+// In-register length-R DFT (R = 2..32), natural order in and out.
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2 (&x)[R]) {
   constexpr int LG = Log2<R>::v;
@@ -67,7 +66,7 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
 #pragma unroll
     for (int b = 0; b < LG; ++b) j |= ((i >> b) & 1) << (LG - 1 - b);
     if (j > i) {
-      float2 t = x[i];
+      const float2 t = x[i];
       x[i] = x[j];
       x[j] = t;
     }
@@ -79,9 +78,9 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
     for (int i = 0; i < R; i += len) {
 #pragma unroll
       for (int j = 0; j < half; ++j) {
-        int k = j * (16 / len);
-        if (INV) k = (16 - k) & 15;
-        const float2 v = twmul16(x[i + j + half], k);
+        int k = j * (32 / len);
+        if (INV) k = (32 - k) & 31;
+        const float2 v = twmul32(x[i + j + half], k);
         const float2 u = x[i + j];
         x[i + j] = cadd(u, v);
         x[i + j + half] = csub(u, v);
@@ -90,222 +89,176 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
   }
 }
 
-template <int N> struct FFTShape;
-template <> struct FFTShape<16> { static constexpr int R1 = 4, R2 = 4; };
-template <> struct FFTShape<32> { static constexpr int R1 = 8, R2 = 4; };
-template <> struct FFTShape<64> { static constexpr int R1 = 8, R2 = 8; };
-template <> struct FFTShape<128> { static constexpr int R1 = 16, R2 = 8; };
+// Quad permutes (DPP quad_perm): value of lane q ^ 1 and of lane q ^ 2.
+__device__ __forceinline__ float dpp_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor2(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float2 quad_xor1(float2 v) { return make_float2(dpp_xor1(v.x), dpp_xor1(v.y)); }
+__device__ __forceinline__ float2 quad_xor2(float2 v) { return make_float2(dpp_xor2(v.x), dpp_xor2(v.y)); }
 
-// Frequency index held at scrambled position p (one axis).
+// Frequency index held at scrambled position p of an N-point axis.
 template <int N>
 __host__ __device__ __forceinline__ int scr_to_freq(int p) {
-  return p / FFTShape<N>::R2 + FFTShape<N>::R1 * (p % FFTShape<N>::R2);
-}
-// Scrambled position holding frequency index k (one axis).
-template <int N>
-__host__ __device__ __forceinline__ int freq_to_scr(int k) {
-  return FFTShape<N>::R2 * (k % FFTShape<N>::R1) + k / FFTShape<N>::R1;
+  const int q = p & 3;
+  return (p >> 2) + (N / 4) * (((q & 1) << 1) | (q >> 1));
 }
 
-// LDS plane of N x N complex values with T threads per workgroup.
-template <int N, int T>
+template <int N>
 struct Plane {
-  static constexpr int R1 = FFTShape<N>::R1, R2 = FFTShape<N>::R2;
-  static constexpr int LD = N + 1;
+  static constexpr int E = N / 4;       // points per lane
+  static constexpr int T = 4 * N;       // threads per workgroup
+  static constexpr int LD = N + 4;      // LDS row stride (float2)
   static constexpr int LDS_FLOAT2 = N * LD;
   float2* s;         // N x LD
-  const float2* tw;  // W_N^e, e in [0, N), in LDS
+  const float2* tw;  // W_N^e, e < N, in LDS
+  int rc;            // row (row phase) or column (column phase) owned by this quad
+  int q;             // lane within the quad
 
-  __device__ __forceinline__ float2& at(int y, int x) { return s[y * LD + x]; }
+  __device__ __forceinline__ Plane(float2* s_, const float2* tw_) : s(s_), tw(tw_) {
+    const int l = threadIdx.x & 63;
+    rc = 16 * (threadIdx.x >> 6) + (l >> 2);
+    q = l & 3;
+  }
 
-  // ---------------- row transforms (along x) ----------------
-  // Forward stage 1 with a fused source: src(y, x) -> natural-order input value.
+  // ----- one 1-D transform along the quad (registers + DPP only) -----
+  __device__ __forceinline__ void fwd(float2 (&x)[E]) {
+    dft<E, false>(x);
+#pragma unroll
+    for (int k = 1; k < E; ++k)
+      if (q) x[k] = cmul(x[k], tw[q * k]);
+    const bool hi = q >= 2, odd = q & 1;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const float2 p = quad_xor2(x[k]);
+      float2 y;
+      if (!hi) y = cadd(x[k], p);
+      else {
+        y = csub(p, x[k]);
+        if (q == 3) y = make_float2(y.y, -y.x);  // * W4^1 = -i
+      }
+      const float2 p2 = quad_xor1(y);
+      x[k] = odd ? csub(p2, y) : cadd(y, p2);
+    }
+  }
+  __device__ __forceinline__ void inv(float2 (&x)[E]) {
+    const bool hi = q >= 2, odd = q & 1;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const float2 p = quad_xor1(x[k]);
+      const float2 a = odd ? csub(p, x[k]) : cadd(x[k], p);
+      const float2 p2 = quad_xor2(a);
+      if (!hi) {
+        float2 w = p2;
+        if (q == 1) w = make_float2(-w.y, w.x);  // * conj(W4^1) = +i
+        x[k] = cadd(a, w);
+      } else {
+        float2 w = a;
+        if (q == 3) w = make_float2(-w.y, w.x);
+        x[k] = csub(p2, w);
+      }
+    }
+#pragma unroll
+    for (int k = 1; k < E; ++k)
+      if (q) x[k] = cmulc(x[k], tw[q * k]);
+    dft<E, true>(x);
+  }
+
+  // ----- transposes through LDS -----
+  // row phase (x[k] at scrambled column 4k+q of row rc) -> column phase (x[j] at row 4j+q
+  // of column rc)
+  __device__ __forceinline__ void rows_to_cols(float2 (&x)[E]) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) s[rc * LD + 4 * k + q] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) x[j] = s[(4 * j + q) * LD + rc];
+  }
+  // column phase (x[j] at row 4j+q of column rc) -> row phase (x[k] at column 4k+q of row rc)
+  __device__ __forceinline__ void cols_to_rows(float2 (&x)[E]) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) s[(4 * j + q) * LD + rc] = x[j];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) x[k] = s[rc * LD + 4 * k + q];
+  }
+
+  // ----- composites -----
+  // natural-order source src(y, x) -> registers (row phase).  The scheduling barrier
+  // every 8 points keeps the compiler from hoisting all of a fused source's loads at
+  // once (which spills at n = 128) while leaving 8 points' loads in flight per lane.
   template <class Src>
-  __device__ __forceinline__ void rows_f1(Src src) {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R2; id += T) {
-      const int r = id / R2, n2 = id % R2;
-      float2 v[R1];
+  __device__ __forceinline__ void load_rows(Src src, float2 (&x)[E]) {
 #pragma unroll
-      for (int j = 0; j < R1; ++j) v[j] = src(r, R2 * j + n2);
-      dft<R1, false>(v);
-      float2* row = s + r * LD;
-#pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) row[R2 * k1 + n2] = (k1 == 0) ? v[0] : cmul(v[k1], tw[n2 * k1]);
+    for (int j = 0; j < E; ++j) {
+      x[j] = src(rc, 4 * j + q);
+      if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
-  __device__ __forceinline__ void rows_f2() {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R1; id += T) {
-      const int r = id / R1, k1 = id % R1;
-      float2* p = s + r * LD + R2 * k1;
-      float2 v[R2];
-#pragma unroll
-      for (int j = 0; j < R2; ++j) v[j] = p[j];
-      dft<R2, false>(v);
-#pragma unroll
-      for (int j = 0; j < R2; ++j) p[j] = v[j];
-    }
-  }
-  __device__ __forceinline__ void rows_iA() {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R1; id += T) {
-      const int r = id / R1, k1 = id % R1;
-      float2* p = s + r * LD + R2 * k1;
-      float2 v[R2];
-#pragma unroll
-      for (int j = 0; j < R2; ++j) v[j] = p[j];
-      dft<R2, true>(v);
-#pragma unroll
-      for (int j = 0; j < R2; ++j) p[j] = v[j];
-    }
-  }
-  // Inverse stage B with a fused sink: dst(y, x, value) receives natural-order output.
+  // registers (row phase, natural) -> sink dst(y, x, v)
   template <class Dst>
-  __device__ __forceinline__ void rows_iB(Dst dst) {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R2; id += T) {
-      const int r = id / R2, n2 = id % R2;
-      const float2* row = s + r * LD;
-      float2 v[R1];
+  __device__ __forceinline__ void store_rows(Dst dst, float2 (&x)[E]) {
 #pragma unroll
-      for (int j = 0; j < R1; ++j) {
-        const float2 a = row[R2 * j + n2];
-        v[j] = (j == 0) ? a : cmulc(a, tw[n2 * j]);
-      }
-      dft<R1, true>(v);
-#pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) dst(r, R2 * n1 + n2, v[n1]);
-    }
+    for (int j = 0; j < E; ++j) dst(rc, 4 * j + q, x[j]);
   }
-  // Inverse stage B written back into LDS.
-  __device__ __forceinline__ void rows_iB_lds() {
-    rows_iB([&](int y, int x, float2 v) { s[y * LD + x] = v; });
+  // row-phase natural data -> column-phase scrambled spectrum (py = 4k+q, px = rc)
+  __device__ __forceinline__ void forward_regs(float2 (&x)[E]) {
+    fwd(x);
+    rows_to_cols(x);
+    fwd(x);
   }
-
-  // ---------------- column transforms (along y) ----------------
-  __device__ __forceinline__ void cols_f1() {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R2; id += T) {
-      const int c = id % N, n2 = id / N;
-      float2 v[R1];
-#pragma unroll
-      for (int j = 0; j < R1; ++j) v[j] = s[(R2 * j + n2) * LD + c];
-      dft<R1, false>(v);
-#pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) s[(R2 * k1 + n2) * LD + c] = (k1 == 0) ? v[0] : cmul(v[k1], tw[n2 * k1]);
-    }
+  // column-phase scrambled spectrum -> row-phase natural data
+  __device__ __forceinline__ void inverse_regs(float2 (&x)[E]) {
+    inv(x);
+    cols_to_rows(x);
+    inv(x);
   }
-  // Forward stage 2; fn(py, px, v) may modify v (scrambled positions) before it is stored.
-  template <class Fn>
-  __device__ __forceinline__ void cols_f2(Fn fn) {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R1; id += T) {
-      const int c = id % N, k1 = id / N;
-      float2 v[R2];
-#pragma unroll
-      for (int j = 0; j < R2; ++j) v[j] = s[(R2 * k1 + j) * LD + c];
-      dft<R2, false>(v);
-#pragma unroll
-      for (int j = 0; j < R2; ++j) {
-        fn(R2 * k1 + j, c, v[j]);
-        s[(R2 * k1 + j) * LD + c] = v[j];
-      }
-    }
-  }
-  // Forward stage 2, pointwise mul(py, px, v) -> v', inverse stage A, all in registers.
-  template <class Mul>
-  __device__ __forceinline__ void cols_f2_mul_iA(Mul mul) {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R1; id += T) {
-      const int c = id % N, k1 = id / N;
-      float2 v[R2];
-#pragma unroll
-      for (int j = 0; j < R2; ++j) v[j] = s[(R2 * k1 + j) * LD + c];
-      dft<R2, false>(v);
-#pragma unroll
-      for (int j = 0; j < R2; ++j) v[j] = mul(R2 * k1 + j, c, v[j]);
-      dft<R2, true>(v);
-#pragma unroll
-      for (int j = 0; j < R2; ++j) s[(R2 * k1 + j) * LD + c] = v[j];
-    }
-  }
-  // Inverse stage A with a fused source: src(py, px) -> scrambled-order input value.
-  template <class Src>
-  __device__ __forceinline__ void cols_iA(Src src) {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R1; id += T) {
-      const int c = id % N, k1 = id / N;
-      float2 v[R2];
-#pragma unroll
-      for (int j = 0; j < R2; ++j) v[j] = src(R2 * k1 + j, c);
-      dft<R2, true>(v);
-#pragma unroll
-      for (int j = 0; j < R2; ++j) s[(R2 * k1 + j) * LD + c] = v[j];
-    }
-  }
-  __device__ __forceinline__ void cols_iB() {
-#pragma unroll 1
-    for (int id = threadIdx.x; id < N * R2; id += T) {
-      const int c = id % N, n2 = id / N;
-      float2 v[R1];
-#pragma unroll
-      for (int j = 0; j < R1; ++j) {
-        const float2 a = s[(R2 * j + n2) * LD + c];
-        v[j] = (j == 0) ? a : cmulc(a, tw[n2 * j]);
-      }
-      dft<R1, true>(v);
-#pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) s[(R2 * n1 + n2) * LD + c] = v[n1];
-    }
-  }
-
-  // Full forward transform from a fused source; result scrambled in LDS; fn as in cols_f2.
+  // forward transform of src; fn(py, px, v&) sees every scrambled spectrum value
   template <class Src, class Fn>
   __device__ __forceinline__ void forward(Src src, Fn fn) {
-    rows_f1(src);
-    __syncthreads();
-    rows_f2();
-    __syncthreads();
-    cols_f1();
-    __syncthreads();
-    cols_f2(fn);
-    __syncthreads();
+    float2 x[E];
+    load_rows(src, x);
+    forward_regs(x);
+#pragma unroll
+    for (int k = 0; k < E; ++k) fn(4 * k + q, rc, x[k]);
   }
-  // Full inverse transform of the scrambled LDS plane into a fused sink (src as in cols_iA).
+  // inverse transform of the scrambled spectrum src(py, px) into dst(y, x, v)
   template <class Src, class Dst>
   __device__ __forceinline__ void inverse(Src src, Dst dst) {
-    cols_iA(src);
-    __syncthreads();
-    cols_iB();
-    __syncthreads();
-    rows_iA();
-    __syncthreads();
-    rows_iB(dst);
+    float2 x[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      x[k] = src(4 * k + q, rc);
+      if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    inverse_regs(x);
+    store_rows(dst, x);
   }
-  // forward -> pointwise -> inverse
+  // in-register forward -> pointwise mul(py, px, v) -> inverse
+  template <class Mul>
+  __device__ __forceinline__ void convolve_regs(float2 (&x)[E], Mul mul) {
+    forward_regs(x);
+#pragma unroll
+    for (int k = 0; k < E; ++k) x[k] = mul(4 * k + q, rc, x[k]);
+    inverse_regs(x);
+  }
   template <class Src, class Mul, class Dst>
   __device__ __forceinline__ void convolve(Src src, Mul mul, Dst dst) {
-    rows_f1(src);
-    __syncthreads();
-    rows_f2();
-    __syncthreads();
-    cols_f1();
-    __syncthreads();
-    cols_f2_mul_iA(mul);
-    __syncthreads();
-    cols_iB();
-    __syncthreads();
-    rows_iA();
-    __syncthreads();
-    rows_iB(dst);
+    float2 x[E];
+    load_rows(src, x);
+    convolve_regs(x, mul);
+    store_rows(dst, x);
   }
 };
 
 // Load the twiddle table (n entries, host-computed in FP64) into LDS.
-template <int N, int T>
+template <int N>
 __device__ __forceinline__ void load_twiddles(float2* tw_lds, const float2* tw_glob) {
-  for (int i = threadIdx.x; i < N; i += T) tw_lds[i] = tw_glob[i];
+  for (int i = threadIdx.x; i < N; i += Plane<N>::T) tw_lds[i] = tw_glob[i];
 }
 
 }  // namespace phx

@@ -13,8 +13,8 @@ import numpy
 _E_REST = 5.1099906e5   # eV
 _HC = 1.23984244e4      # eV * angstrom
 
-# (R1, R2) of the four-step LDS FFT for each plane side (phaser_amd/csrc/phx_fft.h)
-FFT_SHAPES = {16: (4, 4), 32: (8, 4), 64: (8, 8), 128: (16, 8)}
+# plane sides the FFT kernels are built for (phaser_amd/csrc/phx_fft.h)
+FFT_SIZES = (16, 32, 64, 128)
 
 
 def electron_wavelength(energy_ev: float) -> float:
@@ -72,10 +72,15 @@ def make_propagators(sampling, wavelength, thicknesses, bwlim_frac=2 / 3, dtype=
 
 
 def scr_to_freq(n: int) -> numpy.ndarray:
-    """Frequency index held at each scrambled position of an n-point axis."""
-    r1, r2 = FFT_SHAPES[n]
+    """Frequency index held at each scrambled position of an n-point axis.
+
+    Lane q of a quad ends the transform holding X[k + (n/4) bitrev2(q)] at position
+    4k + q, so position p holds frequency p // 4 + (n / 4) * bitrev2(p % 4)."""
+    if n not in FFT_SIZES:
+        raise ValueError(f"unsupported plane side {n}; the FFT kernels support {FFT_SIZES}")
     p = numpy.arange(n)
-    return p // r2 + r1 * (p % r2)
+    q = p % 4
+    return p // 4 + (n // 4) * (((q & 1) << 1) | (q >> 1))
 
 
 def scramble(a: numpy.ndarray) -> numpy.ndarray:

@@ -1,82 +1,101 @@
-"""CPU model of the index scheme used by the LDS FFT kernels (phaser_amd/csrc/phx_fft.h):
-four-step n = R1*R2 transforms, forward natural -> scrambled, inverse scrambled -> natural,
-all stages in place.  The same formulas as the kernel are restated here with numpy so an
-indexing mistake fails on the CPU before any GPU run."""
+"""CPU model of the index scheme of the quad FFT kernels (phaser_amd/csrc/phx_fft.h).
+
+Four lanes (a quad) own one row or column; lane q holds points 4j+q.  Per lane: an
+E = n/4 point DFT over j, a twiddle W_n^(q k), then a DFT_4 across the quad in two
+radix-2 exchange stages (partner q^2, then q^1) written exactly as the kernel writes
+them.  The forward leaves lane q holding X[k + E bitrev2(q)] at position 4k + q; the
+inverse undoes it.  Restating the kernel's butterflies here with numpy makes a sign,
+rotation or index mistake fail on the CPU before any GPU run."""
 import numpy
 import pytest
 
-SHAPES = {16: (4, 4), 32: (8, 4), 64: (8, 8), 128: (16, 8)}
+from phaser_amd.optics import scr_to_freq
 
 
-def _dft(v, inv):
-    return numpy.fft.ifft(v) * len(v) if inv else numpy.fft.fft(v)
-
-
-def rows_f1(s, n):
-    R1, R2 = SHAPES[n]
+def _lane_fwd(lanes, n):
+    """lanes[q] = E-vector of lane q (points 4j+q) -> lanes after fwd()."""
+    E = n // 4
     tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
-    for r in range(n):
-        for n2 in range(R2):
-            idx = R2 * numpy.arange(R1) + n2
-            v = _dft(s[r, idx], False)
-            v = v * tw[(n2 * numpy.arange(R1))]
-            s[r, R2 * numpy.arange(R1) + n2] = v
+    x = [numpy.fft.fft(lanes[q]) for q in range(4)]
+    x = [x[q] * tw[q * numpy.arange(E)] for q in range(4)]
+    y = [None] * 4
+    for q in range(4):
+        p = x[q ^ 2]
+        if q < 2:
+            y[q] = x[q] + p
+        else:
+            y[q] = p - x[q]
+            if q == 3:
+                y[q] = y[q] * -1j
+    out = [None] * 4
+    for q in range(4):
+        p2 = y[q ^ 1]
+        out[q] = (p2 - y[q]) if (q & 1) else (y[q] + p2)
+    return out
 
 
-def rows_f2(s, n, inv=False):
-    R1, R2 = SHAPES[n]
-    for r in range(n):
-        for k1 in range(R1):
-            idx = R2 * k1 + numpy.arange(R2)
-            s[r, idx] = _dft(s[r, idx], inv)
-
-
-def rows_iB(s, n):
-    R1, R2 = SHAPES[n]
+def _lane_inv(lanes, n):
+    E = n // 4
     tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
-    for r in range(n):
-        for n2 in range(R2):
-            j = numpy.arange(R1)
-            v = s[r, R2 * j + n2] * numpy.conj(tw[n2 * j])
-            s[r, R2 * j + n2] = _dft(v, True)
+    a = [None] * 4
+    for q in range(4):
+        p = lanes[q ^ 1]
+        a[q] = (p - lanes[q]) if (q & 1) else (lanes[q] + p)
+    x = [None] * 4
+    for q in range(4):
+        p2 = a[q ^ 2]
+        if q < 2:
+            x[q] = a[q] + (p2 * 1j if q == 1 else p2)
+        else:
+            x[q] = p2 - (a[q] * 1j if q == 3 else a[q])
+    x = [x[q] * numpy.conj(tw[q * numpy.arange(E)]) for q in range(4)]
+    return [numpy.fft.ifft(x[q]) * E for q in range(4)]
+
+
+def _axis_fwd(v, n):
+    """natural vector -> vector in scrambled positions (position 4k+q from lane q, k)."""
+    out = numpy.empty(n, complex)
+    lanes = _lane_fwd([v[q::4] for q in range(4)], n)
+    for q in range(4):
+        out[q::4] = lanes[q]
+    return out
+
+
+def _axis_inv(v, n):
+    out = numpy.empty(n, complex)
+    lanes = _lane_inv([v[q::4] for q in range(4)], n)
+    for q in range(4):
+        out[q::4] = lanes[q]
+    return out
 
 
 def forward(a, n):
-    s = a.astype(numpy.complex128).copy()
-    rows_f1(s, n); rows_f2(s, n)
-    s = s.T.copy(); rows_f1(s, n); rows_f2(s, n); s = s.T.copy()
-    return s
+    """rows (row phase) -> transpose through LDS -> columns (column phase)."""
+    s = numpy.stack([_axis_fwd(a[r], n) for r in range(n)])          # s[r, scrambled col]
+    return numpy.stack([_axis_fwd(s[:, c], n) for c in range(n)]).T  # [scr row, scr col]
 
 
 def inverse(s, n):
-    s = s.copy()
-    s = s.T.copy(); rows_f2(s, n, inv=True); rows_iB(s, n); s = s.T.copy()
-    rows_f2(s, n, inv=True); rows_iB(s, n)
-    return s
+    t = numpy.stack([_axis_inv(s[:, c], n) for c in range(n)]).T     # columns first
+    return numpy.stack([_axis_inv(t[r], n) for r in range(n)])
 
 
-def scr_to_freq(p, n):
-    R1, R2 = SHAPES[n]
-    return p // R2 + R1 * (p % R2)
-
-
-def freq_to_scr(k, n):
-    R1, R2 = SHAPES[n]
-    return R2 * (k % R1) + k // R1
-
-
-@pytest.mark.parametrize('n', [16, 32, 64])
-def test_four_step_scrambled_forward_and_natural_inverse(n):
+@pytest.mark.parametrize('n', [16, 32, 64, 128])
+def test_quad_scheme_scrambled_forward_and_natural_inverse(n):
     rng = numpy.random.default_rng(n)
     a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     X = numpy.fft.fft2(a)
+    f = scr_to_freq(n)
+    assert sorted(f) == list(range(n))
     s = forward(a, n)
-    p = numpy.arange(n)
-    k = scr_to_freq(p, n)
-    numpy.testing.assert_allclose(s, X[numpy.ix_(k, k)], atol=1e-9)
-    numpy.testing.assert_array_equal(freq_to_scr(k, n), p)
-    back = inverse(s, n) / (n * n)
-    numpy.testing.assert_allclose(back, a, atol=1e-12)
+    numpy.testing.assert_allclose(s, X[numpy.ix_(f, f)], atol=1e-9 * n * n)
+    numpy.testing.assert_allclose(inverse(s, n) / (n * n), a, atol=1e-12)
+
+
+def test_scrambled_order_small_case_by_hand():
+    # n = 16, E = 4: lane 1 holds X[k + 8] (bitrev2(1) = 2), lane 2 holds X[k + 4]
+    f = scr_to_freq(16)
+    assert list(f[:8]) == [0, 8, 4, 12, 1, 9, 5, 13]
 
 
 def test_pointwise_between_transforms_equals_fft_convolution():
@@ -84,7 +103,11 @@ def test_pointwise_between_transforms_equals_fft_convolution():
     rng = numpy.random.default_rng(5)
     a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     h = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
-    k = scr_to_freq(numpy.arange(n), n)
-    h_scr = h[numpy.ix_(k, k)]   # constant stored in scrambled order
-    out = inverse(forward(a, n) * h_scr, n) / (n * n)
+    f = scr_to_freq(n)
+    out = inverse(forward(a, n) * h[numpy.ix_(f, f)], n) / (n * n)
     numpy.testing.assert_allclose(out, numpy.fft.ifft2(numpy.fft.fft2(a) * h), atol=1e-12)
+
+
+def test_unsupported_side_raises():
+    with pytest.raises(ValueError):
+        scr_to_freq(48)
