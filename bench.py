@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-groups", type=int, default=8)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the run, copy a device buffer of this many MiB once (a copy kernel of "
+                         "known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
@@ -224,6 +227,13 @@ def main():
         "detector_error_first_last": [errs[0], errs[-1]],
     }
     print(json.dumps(line), flush=True)
+    if args.calib_copy_mb:
+        src = torch.ones(args.calib_copy_mb << 18, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_copy_mb} MiB read + {args.calib_copy_mb} MiB written")
 
 
 class _NullObs:

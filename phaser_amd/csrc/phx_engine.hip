@@ -124,8 +124,8 @@ __global__ void __launch_bounds__(256) group_obj_mag_k(phx_group_desc d) {
 }
 
 // psi0 = ifft2(fft2(P) * filt_b), illumination, norm, optional slice-0 step.
-template <int N, int T>
-__global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum, int propagate) {
+template <int N, int T, bool ILLUM>
+__global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int propagate) {
   const int m = blockIdx.x, b = blockIdx.y;
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
@@ -158,8 +158,8 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum,
     const float a = abs2f(v);
     nrm += a;
     const bool inb = c.inb(yy, xx);
-    if (illum && inb) atomic_add_f32(&gpm0[c.idx(yy, xx)], a);
-    x[j] = cmul(v, inb ? obj0[c.idx(yy, xx)] : make_float2(1.f, 0.f));
+    if (ILLUM && inb) atomic_add_f32(&gpm0[c.idx(yy, xx)], a);
+    x[j] = cmul(v, obj_at(obj0, c, yy, xx));
   }
   const double tot = block_sum_f64(nrm, red_lds_);
   if (threadIdx.x == 0) {
@@ -173,8 +173,8 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int illum,
   pl.store_rows([&](int yy, int xx, float2 v) { psi1[yy * N + xx] = cscale(v, inv_nn); }, x);
 }
 
-template <int N, int T>
-__global__ void __launch_bounds__(T) forward_slice_k(phx_group_desc d, int i, int illum) {
+template <int N, int T, bool ILLUM>
+__global__ void __launch_bounds__(T) forward_slice_k(phx_group_desc d, int i) {
   const int m = blockIdx.x, b = blockIdx.y;
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
@@ -192,16 +192,16 @@ __global__ void __launch_bounds__(T) forward_slice_k(phx_group_desc d, int i, in
       [&](int y, int x) {
         const float2 v = psi_i[y * N + x];
         const bool inb = c.inb(y, x);
-        if (illum && inb) atomic_add_f32(&gpm_i[c.idx(y, x)], abs2f(v));
-        return cmul(v, inb ? obj_i[c.idx(y, x)] : make_float2(1.f, 0.f));
+        if (ILLUM && inb) atomic_add_f32(&gpm_i[c.idx(y, x)], abs2f(v));
+        return cmul(v, obj_at(obj_i, c, y, x));
       },
       [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); },
       [&](int y, int x, float2 v) { psi_o[y * N + x] = cscale(v, inv_nn); });
 }
 
 // Exit wave spectrum and model intensity (sum over modes).
-template <int N, int T>
-__global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d, int illum) {
+template <int N, int T, bool ILLUM>
+__global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d) {
   const int m = blockIdx.x, b = blockIdx.y;
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
@@ -220,8 +220,8 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d, int illum)
       [&](int y, int x) {
         const float2 v = psi_s[y * N + x];
         const bool inb = c.inb(y, x);
-        if (illum && inb) atomic_add_f32(&gpm_s[c.idx(y, x)], abs2f(v));
-        return cmul(v, inb ? obj_s[c.idx(y, x)] : make_float2(1.f, 0.f));
+        if (ILLUM && inb) atomic_add_f32(&gpm_s[c.idx(y, x)], abs2f(v));
+        return cmul(v, obj_at(obj_s, c, y, x));
       },
       [&](int py, int px, float2& v) {
         const int e = py * N + px;
@@ -237,8 +237,8 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d, int illum)
 //  FM_GRAD_AMPLITUDE / FM_GRAD_POISSON: gradient of the amplitude (noise_models.py:29-45)
 //    or Poisson (:86-102) loss w.r.t. the exit wave, in torch's convention
 //    (2 dL/dconj(a)), times params[P_LOSS_SCALE]; the group loss (FP64) goes to red.loss.
-template <int N, int T>
-__global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int calc_error, int mode) {
+template <int N, int T, int MODE>
+__global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int calc_error) {
   const int m = blockIdx.x, b = blockIdx.y;
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
@@ -252,45 +252,45 @@ __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int cal
   const float* mask = d.mask_s;
   const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
   float2* chi = (float2*)d.chi_a + ((size_t)b * M + m) * NN;
-  const bool err = calc_error && m == 0 && mode == FM_MODULUS;
-  const bool want_loss = mode != FM_MODULUS && m == 0;
+  // the error (modulus mode) or the loss (gradient modes) is summed by mode 0 only
+  const bool err = MODE == FM_MODULUS ? (calc_error && m == 0) : (m == 0);
   const float var = d.params[P_VAR];
-  const float gscale = mode != FM_MODULUS ? d.params[P_LOSS_SCALE] : 1.0f;
-  double e2 = 0.0;
+  const float gscale = MODE != FM_MODULUS ? d.params[P_LOSS_SCALE] : 1.0f;
+  float e2 = 0.f;  // per-lane FP32 partial over its n/4 points; block sum in FP64
   pl.inverse(
       [&](int py, int px) {
         const int e = py * N + px;
-        const float Iv = I[e], Dv = D[e];
+        const float Iv = I[e], Dv = D[e], mk = mask[e];
+        const float2 Fv = F[e];
         const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
-        float u;
-        if (mode == FM_MODULUS) {
-          if (err) {
-            const float df = Iv - Dv;
-            const float sq = df * df;
-            if (!isnan(sq)) e2 += (double)sq;
-          }
-          u = sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f;
-          u *= mask[e];
-        } else if (mode == FM_GRAD_AMPLITUDE) {
+        float u, t;
+        if (MODE == FM_MODULUS) {
+          const float df = Iv - Dv;
+          t = df * df;  // unmasked (solvers.py:237), NaN-skipping (nansum)
+          t = isnan(t) ? 0.f : t;
+          u = (sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f) * mk;
+        } else if (MODE == FM_GRAD_AMPLITUDE) {
           const float si = sqrtf(Iv + off);
-          const float s = sqrtf(Dp + off) - si - eps;
-          if (want_loss) e2 += (double)(mask[e] * s * s);
-          u = -4.0f * mask[e] * s / (var * si) * gscale;
+          const float sd = sqrtf(Dp + off) - si - eps;
+          t = mk * sd * sd;
+          u = -4.0f * mk * sd / (var * si) * gscale;
         } else {
-          if (want_loss) e2 += (double)(mask[e] * (Iv + eps + Dp * (logf(Dp + eps) - logf(Iv + eps) - 1.0f)));
-          u = 2.0f * mask[e] * (1.0f - Dp / (Iv + eps)) * gscale;
+          t = mk * (Iv + eps + Dp * (logf(Dp + eps) - logf(Iv + eps) - 1.0f));
+          u = 2.0f * mk * (1.0f - Dp / (Iv + eps)) * gscale;
         }
-        return cscale(F[e], u);
+        e2 += t;
+        return cscale(Fv, u);
       },
-      [&](int y, int x, float2 v) { chi[y * N + x] = cscale(v, inv_nn); });
+      [&](int y, int x, float2 v) { chi[y * N + x] = cscale(v, inv_nn); },
+      [&] { __asm__ volatile("" : "+v"(e2)); });
   if (err) {
-    const double tot = block_sum_f64(e2, red_lds_);
-    if (threadIdx.x == 0) d.errors[b] = (float)sqrt(tot);
-  }
-  if (want_loss) {
     double tot = block_sum_f64(e2, red_lds_);
-    if (mode == FM_GRAD_AMPLITUDE) tot *= 2.0 / (double)var;
-    if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(d.n_slices, d.bmax, M).loss], tot);
+    if (MODE == FM_MODULUS) {
+      if (threadIdx.x == 0) d.errors[b] = (float)sqrt(tot);
+    } else {
+      if (MODE == FM_GRAD_AMPLITUDE) tot *= 2.0 / (double)var;
+      if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(d.n_slices, d.bmax, M).loss], tot);
+    }
   }
 }
 
@@ -335,25 +335,27 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
       const float2 t = cmul(cmulc(cv, ps), ps);  // (chi conj(psi)) psi
       numO += cv.x * t.x + cv.y * t.y;
       denO += abs2f(t);
-      // keeps the sums in the load phase: sunk to their use after the transform they
-      // would hold every point's inputs live and spill at n = 128
-      __asm__ volatile("" : "+v"(numO), "+v"(denO));
     }
     const float2 dP = cmulc(cv, o);  // chi * conj(O)
     if (PROBE) {
       const float2 t2 = cmul(dP, o);
       numP += cv.x * t2.x + cv.y * t2.y;
       denP += abs2f(t2);
-      __asm__ volatile("" : "+v"(numP), "+v"(denP));
     }
     return dP;
+  };
+  // keeps the sums in the load phase, one group of points at a time: sunk to their use
+  // after the transform they would hold every point's inputs live and spill at n = 128
+  auto pin = [&] {
+    if (UPD_OBJ) __asm__ volatile("" : "+v"(numO), "+v"(denO));
+    if (PROBE) __asm__ volatile("" : "+v"(numP), "+v"(denP));
   };
 
   if (KIND == BK_PROPAGATE) {
     float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
     const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
     pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
-                [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); });
+                [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); }, pin);
   } else if (PROBE) {
     // per-plane fft2(dP) conj(filt_b) into the (free by now) farfield buffer; the sum
     // over positions is probe_spectrum_reduce_k
@@ -361,7 +363,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
     float2* Fp = (float2*)d.farfield + ((size_t)b * M + m) * NN;
     pl.forward(src, [&](int py, int px, float2& v) {
       Fp[py * N + px] = cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]));
-    });
+    }, pin);
   } else {
     for (int e = threadIdx.x; e < NN; e += T) (void)src(e / N, e % N);
   }
@@ -909,8 +911,13 @@ int phx_shift_probes(const phx_group_desc* d, int illum, int propagate, hipStrea
   if (r) return r;
   if (!d->probe_k || (illum && !d->gpm)) return PHX_EINVAL;
   if (propagate && d->n_slices < 2) return PHX_EINVAL;
-#define L(N, T) \
-  hipLaunchKernelGGL((shift_probes_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, illum, propagate)
+#define L(N, T)                                                                                          \
+  do {                                                                                                   \
+    if (illum)                                                                                           \
+      hipLaunchKernelGGL((shift_probes_k<N, T, true>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, propagate);  \
+    else                                                                                                 \
+      hipLaunchKernelGGL((shift_probes_k<N, T, false>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, propagate); \
+  } while (0)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
@@ -921,7 +928,13 @@ int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t
   int r = check_desc(d);
   if (r) return r;
   if (slice < 0 || slice > d->n_slices - 2 || (illum && !d->gpm)) return PHX_EINVAL;
-#define L(N, T) hipLaunchKernelGGL((forward_slice_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, illum)
+#define L(N, T)                                                                                          \
+  do {                                                                                                   \
+    if (illum)                                                                                           \
+      hipLaunchKernelGGL((forward_slice_k<N, T, true>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice);     \
+    else                                                                                                 \
+      hipLaunchKernelGGL((forward_slice_k<N, T, false>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice);    \
+  } while (0)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
@@ -934,12 +947,18 @@ int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t
   if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a || (illum && !d->gpm) ||
       (calc_error && !d->errors))
     return PHX_EINVAL;
-#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, illum)
+#define L(N, T)                                                                                          \
+  do {                                                                                                   \
+    if (illum)                                                                                           \
+      hipLaunchKernelGGL((farfield_fwd_k<N, T, true>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d);             \
+    else                                                                                                 \
+      hipLaunchKernelGGL((farfield_fwd_k<N, T, false>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d);            \
+  } while (0)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
 #define L(N, T) \
-  hipLaunchKernelGGL((farfield_update_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error, 0)
+  hipLaunchKernelGGL((farfield_update_k<N, T, FM_MODULUS>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
@@ -1056,12 +1075,18 @@ int phx_farfield_gradient(const phx_group_desc* d, int poisson, hipStream_t stre
   int r = check_desc(d);
   if (r) return r;
   if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a) return PHX_EINVAL;
-#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0)
+#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T, false>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
   const int mode = poisson ? (int)FM_GRAD_POISSON : (int)FM_GRAD_AMPLITUDE;
-#define L(N, T) hipLaunchKernelGGL((farfield_update_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0, mode)
+#define L(N, T)                                                                                                  \
+  do {                                                                                                           \
+    if (mode == FM_GRAD_POISSON)                                                                                 \
+      hipLaunchKernelGGL((farfield_update_k<N, T, FM_GRAD_POISSON>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0);   \
+    else                                                                                                         \
+      hipLaunchKernelGGL((farfield_update_k<N, T, FM_GRAD_AMPLITUDE>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0); \
+  } while (0)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
@@ -1109,7 +1134,7 @@ int phx_exit_intensity(const phx_group_desc* d, float* out, hipStream_t stream) 
   int r = check_desc(d);
   if (r) return r;
   if (!out || !d->farfield || !d->intensity) return PHX_EINVAL;
-#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0)
+#define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T, false>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();

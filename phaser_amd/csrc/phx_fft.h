@@ -24,6 +24,12 @@
 #pragma once
 #include "phx_common.h"
 
+// Points per lane whose loads are issued together in a load phase (a scheduling
+// barrier after every PHX_LOAD_PACE points).
+#ifndef PHX_LOAD_PACE
+#define PHX_LOAD_PACE 8
+#endif
+
 namespace phx {
 
 // exp(-2 pi i k / 32), k in [0, 32)
@@ -124,45 +130,42 @@ struct Plane {
   }
 
   // ----- one 1-D transform along the quad (registers + DPP only) -----
+  // The DFT-4 across the quad is two radix-2 exchanges (partner q^2, then q^1), each
+  // written as one FMA per component with a per-lane sign (+-1 is exact), and the one
+  // rotation of the radix-2x2 split (by -i on lane 3 in the forward, +i in the inverse,
+  // applied before the exchange that reads it) as a select.  No lane-dependent branches:
+  // a quad's lanes run the same instructions.
   __device__ __forceinline__ void fwd(float2 (&x)[E]) {
     dft<E, false>(x);
+    // tw[0] = 1 exactly, so lane 0 multiplies by 1
 #pragma unroll
-    for (int k = 1; k < E; ++k)
-      if (q) x[k] = cmul(x[k], tw[q * k]);
-    const bool hi = q >= 2, odd = q & 1;
+    for (int k = 1; k < E; ++k) x[k] = cmul(x[k], tw[q * k]);
+    const float s1 = q < 2 ? 1.f : -1.f;   // y = x + p (q < 2), p - x (q >= 2)
+    const float s2 = (q & 1) ? -1.f : 1.f; // out = y + p (even), p - y (odd)
+    const bool r3 = q == 3;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const float2 p = quad_xor2(x[k]);
-      float2 y;
-      if (!hi) y = cadd(x[k], p);
-      else {
-        y = csub(p, x[k]);
-        if (q == 3) y = make_float2(y.y, -y.x);  // * W4^1 = -i
-      }
+      const float yr = fmaf(s1, x[k].x, p.x), yi = fmaf(s1, x[k].y, p.y);
+      const float2 y = make_float2(r3 ? yi : yr, r3 ? -yr : yi);  // lane 3: * W4^1 = -i
       const float2 p2 = quad_xor1(y);
-      x[k] = odd ? csub(p2, y) : cadd(y, p2);
+      x[k] = make_float2(fmaf(s2, y.x, p2.x), fmaf(s2, y.y, p2.y));
     }
   }
   __device__ __forceinline__ void inv(float2 (&x)[E]) {
-    const bool hi = q >= 2, odd = q & 1;
+    const float s1 = q < 2 ? 1.f : -1.f;
+    const float s2 = (q & 1) ? -1.f : 1.f;
+    const bool r3 = q == 3;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const float2 p = quad_xor1(x[k]);
-      const float2 a = odd ? csub(p, x[k]) : cadd(x[k], p);
+      const float ar = fmaf(s2, x[k].x, p.x), ai = fmaf(s2, x[k].y, p.y);
+      const float2 a = make_float2(r3 ? -ai : ar, r3 ? ar : ai);  // lane 3: * conj(W4^1) = +i
       const float2 p2 = quad_xor2(a);
-      if (!hi) {
-        float2 w = p2;
-        if (q == 1) w = make_float2(-w.y, w.x);  // * conj(W4^1) = +i
-        x[k] = cadd(a, w);
-      } else {
-        float2 w = a;
-        if (q == 3) w = make_float2(-w.y, w.x);
-        x[k] = csub(p2, w);
-      }
+      x[k] = make_float2(fmaf(s1, a.x, p2.x), fmaf(s1, a.y, p2.y));
     }
 #pragma unroll
-    for (int k = 1; k < E; ++k)
-      if (q) x[k] = cmulc(x[k], tw[q * k]);
+    for (int k = 1; k < E; ++k) x[k] = cmulc(x[k], tw[q * k]);
     dft<E, true>(x);
   }
 
@@ -188,16 +191,24 @@ struct Plane {
   }
 
   // ----- composites -----
-  // natural-order source src(y, x) -> registers (row phase).  The scheduling barrier
-  // every 8 points keeps the compiler from hoisting all of a fused source's loads at
-  // once (which spills at n = 128) while leaving 8 points' loads in flight per lane.
-  template <class Src>
-  __device__ __forceinline__ void load_rows(Src src, float2 (&x)[E]) {
+  // natural-order source src(y, x) -> registers (row phase).  Points are taken in groups
+  // of PHX_LOAD_PACE: the loads of a group are issued together, then pin() runs (a
+  // kernel passes one that fixes its running sums in place with an empty asm, so the
+  // compiler can neither sink the sums past the transform nor wait point by point).
+  template <class Src, class Pin>
+  __device__ __forceinline__ void load_rows(Src src, float2 (&x)[E], Pin pin) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       x[j] = src(rc, 4 * j + q);
-      if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      if ((j % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || j == E - 1) {
+        pin();
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
+  }
+  template <class Src>
+  __device__ __forceinline__ void load_rows(Src src, float2 (&x)[E]) {
+    load_rows(src, x, [] {});
   }
   // registers (row phase, natural) -> sink dst(y, x, v)
   template <class Dst>
@@ -218,25 +229,36 @@ struct Plane {
     inv(x);
   }
   // forward transform of src; fn(py, px, v&) sees every scrambled spectrum value
-  template <class Src, class Fn>
-  __device__ __forceinline__ void forward(Src src, Fn fn) {
+  template <class Src, class Fn, class Pin>
+  __device__ __forceinline__ void forward(Src src, Fn fn, Pin pin) {
     float2 x[E];
-    load_rows(src, x);
+    load_rows(src, x, pin);
     forward_regs(x);
 #pragma unroll
     for (int k = 0; k < E; ++k) fn(4 * k + q, rc, x[k]);
   }
+  template <class Src, class Fn>
+  __device__ __forceinline__ void forward(Src src, Fn fn) {
+    forward(src, fn, [] {});
+  }
   // inverse transform of the scrambled spectrum src(py, px) into dst(y, x, v)
-  template <class Src, class Dst>
-  __device__ __forceinline__ void inverse(Src src, Dst dst) {
+  template <class Src, class Dst, class Pin>
+  __device__ __forceinline__ void inverse(Src src, Dst dst, Pin pin) {
     float2 x[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       x[k] = src(4 * k + q, rc);
-      if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      if ((k % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || k == E - 1) {
+        pin();
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     inverse_regs(x);
     store_rows(dst, x);
+  }
+  template <class Src, class Dst>
+  __device__ __forceinline__ void inverse(Src src, Dst dst) {
+    inverse(src, dst, [] {});
   }
   // in-register forward -> pointwise mul(py, px, v) -> inverse
   template <class Mul>
@@ -246,12 +268,16 @@ struct Plane {
     for (int k = 0; k < E; ++k) x[k] = mul(4 * k + q, rc, x[k]);
     inverse_regs(x);
   }
-  template <class Src, class Mul, class Dst>
-  __device__ __forceinline__ void convolve(Src src, Mul mul, Dst dst) {
+  template <class Src, class Mul, class Dst, class Pin>
+  __device__ __forceinline__ void convolve(Src src, Mul mul, Dst dst, Pin pin) {
     float2 x[E];
-    load_rows(src, x);
+    load_rows(src, x, pin);
     convolve_regs(x, mul);
     store_rows(dst, x);
+  }
+  template <class Src, class Mul, class Dst>
+  __device__ __forceinline__ void convolve(Src src, Mul mul, Dst dst) {
+    convolve(src, mul, dst, [] {});
   }
 };
 

@@ -16,38 +16,23 @@ def _lane_fwd(lanes, n):
     """lanes[q] = E-vector of lane q (points 4j+q) -> lanes after fwd()."""
     E = n // 4
     tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
-    x = [numpy.fft.fft(lanes[q]) for q in range(4)]
-    x = [x[q] * tw[q * numpy.arange(E)] for q in range(4)]
-    y = [None] * 4
+    x = [numpy.fft.fft(lanes[q]) * tw[q * numpy.arange(E)] for q in range(4)]
+    y = []
     for q in range(4):
-        p = x[q ^ 2]
-        if q < 2:
-            y[q] = x[q] + p
-        else:
-            y[q] = p - x[q]
-            if q == 3:
-                y[q] = y[q] * -1j
-    out = [None] * 4
-    for q in range(4):
-        p2 = y[q ^ 1]
-        out[q] = (p2 - y[q]) if (q & 1) else (y[q] + p2)
-    return out
+        s1 = 1.0 if q < 2 else -1.0          # y = s1 x + partner(q^2)
+        yy = s1 * x[q] + x[q ^ 2]
+        y.append(yy * -1j if q == 3 else yy)  # lane 3 rotated before the q^1 exchange
+    return [(-1.0 if q & 1 else 1.0) * y[q] + y[q ^ 1] for q in range(4)]
 
 
 def _lane_inv(lanes, n):
     E = n // 4
     tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
-    a = [None] * 4
+    a = []
     for q in range(4):
-        p = lanes[q ^ 1]
-        a[q] = (p - lanes[q]) if (q & 1) else (lanes[q] + p)
-    x = [None] * 4
-    for q in range(4):
-        p2 = a[q ^ 2]
-        if q < 2:
-            x[q] = a[q] + (p2 * 1j if q == 1 else p2)
-        else:
-            x[q] = p2 - (a[q] * 1j if q == 3 else a[q])
+        aa = (-1.0 if q & 1 else 1.0) * lanes[q] + lanes[q ^ 1]
+        a.append(aa * 1j if q == 3 else aa)
+    x = [(1.0 if q < 2 else -1.0) * a[q] + a[q ^ 2] for q in range(4)]
     x = [x[q] * numpy.conj(tw[q * numpy.arange(E)]) for q in range(4)]
     return [numpy.fft.ifft(x[q]) * E for q in range(4)]
 
