@@ -45,8 +45,16 @@ def kernel_bytes(name, B, M, S, NN, HW):
         # read psi_{S-1}, O cutout; write spectrum + intensity; read them back with the
         # pattern; write chi
         'farfield': B * (2 * M * plane + plane + 2 * r * NN + 2 * M * plane + r * NN + r * NN),
-        # gather chi_i and psi_i of every slot; read-modify-write of the covered object
-        'lsqml_object_update': B * 2 * M * plane + 2 * c * min(HW, B * NN),
+        # all S slices: gather chi_i and psi_i of every slot; read-modify-write of the
+        # covered object
+        'lsqml_object_update_slices': S * (B * 2 * M * plane + 2 * c * min(HW, B * NN)),
+        # fused forward: probe spectra in; every psi_i and the exit spectrum out; the S
+        # object cutouts and intensity per position; the propagators
+        'group_forward': B * (M * (S + 1) * plane + S * plane + r * NN) + M * plane + (S - 1) * plane,
+        # fused LSQ-ML backward: exit spectrum in, every chi_i out, every psi_i in (alpha
+        # sums), the probe spectrum out; per position the intensity, pattern and S
+        # cutouts; the propagators and the mask
+        'lsqml_backward_sweep': B * (M * (2 * S + 2) * plane + S * plane + 2 * r * NN) + (S - 1) * plane + r * NN,
     }
     return per_launch[name]
 
@@ -158,7 +166,7 @@ def main():
     n_pos = int(numpy.prod(state.scan.shape[:-1]))
     B = int(numpy.ceil(n_pos / G))
     ws = GroupWorkspace(state, pats.patterns, pats.pattern_mask, props, noise_offset=0.0, noise_eps=1e-3,
-                        n_groups=G, bmax=B)
+                        n_groups=G, bmax=B, keep_chi=True)
     ws.set_positions(state.object.sampling, state.scan)
     sim = SimulationState(state=state, noise_model=plan.noise_model, workspace=ws)
     solver = LSQMLSolver(plan, plan.solver, use_graphs=not args.no_graphs)
@@ -193,7 +201,7 @@ def main():
     ws.set_groups(list(groups.iter(state.scan, 999, True)))
     kt = time_kernels(ws, groups.iter, torch, G)
     modeled = ('forward_slice', 'lsqml_backward', 'lsqml_backward_slice0', 'shift_probes', 'farfield',
-               'lsqml_object_update')
+               'lsqml_object_update_slices', 'group_forward', 'lsqml_backward_sweep')
     dom = max((k for k in kt if k in modeled), key=lambda k: kt[k]['total_ms'])
     NN, M, S, HW = ws.NN, ws.M, ws.S, ws.H * ws.W
     alg = kernel_bytes(dom, B, M, S, NN, HW)

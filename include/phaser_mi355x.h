@@ -116,6 +116,9 @@ typedef struct phx_group_desc {
   int32_t pad_;
   void* grad_obj;            /* (S, H, W) complex64 object gradient accumulator (gradient engine) */
   void* grad_probe;          /* (M, n, n) complex64 probe gradient (gradient engine) */
+  void* chi_slices;          /* (S-1, bmax, M, n, n) complex64 or NULL.  When set, the
+                                backward steps keep every slice's chi (chi_{S-1} stays in
+                                chi_a) so all object updates of a group can run at once */
 } phx_group_desc;
 
 /* Doubles needed by phx_group_desc.red. */
@@ -148,6 +151,28 @@ int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t
  * calc_error != 0, errors[b] = sqrt(nansum((I - D)^2)) (solvers.py:237, FP64 sum). */
 int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t stream);
 
+/* Fused forward model of a group (simulation.py:241-285, solvers.py:202-236): one launch
+ * carries each (slot, mode) plane through every slice with the wave in registers:
+ * psi[0] = ifft2(fft2(probe) * shift_filter(b)), psi[i+1] = ifft2(fft2(psi[i] * O_i) *
+ * prop_i) stored for every i, then farfield = FFT(psi[S-1] * O_{S-1}) and intensity +=
+ * |farfield|^2 / n^2 (intensity zeroed by phx_group_begin), plus the per-slot norm
+ * sum_m |psi[0]|^2.  Replaces phx_shift_probes + phx_forward_slice + the first half of
+ * phx_farfield.  Needs psi_slots covering every slice. */
+int phx_group_forward(const phx_group_desc* d, hipStream_t stream);
+/* The far-field modulus update alone (the second half of phx_farfield): chi_a =
+ * ifft2((sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask farfield), errors[b] for mode 0 when
+ * calc_error (solvers.py:231-240, noise_models.py:47-63). */
+int phx_farfield_update(const phx_group_desc* d, int calc_error, hipStream_t stream);
+/* Fused LSQ-ML backward sweep (solvers.py:231-271) after phx_group_forward: the modulus
+ * update, then for i = S-1 .. 0 with chi_i carried in registers: chi_i stored (chi_a for
+ * i = S-1, chi_slices otherwise), the FP64 alpha_O sums when update_object, and
+ * chi_{i-1} = ifft2(fft2(chi_i conj(O_i)) conj(prop_{i-1})); at slice 0 with
+ * update_probe, each slot's fft2(chi_0 conj(O_0)) conj(filt_b) into farfield and the
+ * alpha_P sums.  Object updates (phx_lsqml_object_update_slices) and the probe update
+ * follow.  Needs chi_slices when S > 1. */
+int phx_lsqml_backward_sweep(const phx_group_desc* d, int update_object, int update_probe, int calc_error,
+                             hipStream_t stream);
+
 /* LSQ-ML backward step at slice i (solvers.py:242-271).  Reads chi from chi_a (even
  * steps from the exit) or chi_b; when update_object, accumulates the FP64 alpha_O sums;
  * for i > 0 writes the back-propagated chi to the other buffer; for i == 0 and
@@ -155,12 +180,18 @@ int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t
  * sums. */
 int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
                        hipStream_t stream);
-/* LSQ-ML object update of slice i (solvers.py:213-215, 248-257), run after
- * phx_lsqml_backward(i) and before the step of slice i-1:
+/* LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
  * O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16) with
  * dO = sum_{b,m} scatter(chi_i conj(psi_i)) and gpm = sum_{b,m} scatter(|psi_i|^2), both
- * gathered per 32x32 object tile in position order (no atomics, deterministic). */
+ * gathered per 16x16 object tile in position order (no atomics, deterministic).
+ * Without chi_slices it runs after phx_lsqml_backward(i) and before the step of slice
+ * i-1 (chi ping-pongs).  With chi_slices it may run after the whole backward sweep: the
+ * sweep reads only the pre-update cutouts (solvers.py:246-260), so deferring every slice's
+ * update to the end of the group gives the same result. */
 int phx_lsqml_object_update(const phx_group_desc* d, int slice, hipStream_t stream);
+/* The updates of slices first .. first+count-1 in one launch (needs chi_slices when
+ * count > 1). */
+int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count, hipStream_t stream);
 /* d_probe[m] = sum_b farfield[b][m] (position order): the probe-update spectrum. */
 int phx_probe_spectrum_reduce(const phx_group_desc* d, hipStream_t stream);
 /* P += beta_P (sum_b alpha_P) ifft2(d_probe)/(gom + reg) * gom/(gom + 1e-16); d_probe = 0. */

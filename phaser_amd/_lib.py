@@ -31,6 +31,7 @@ class GroupDesc(ctypes.Structure):
         ("gpm", c_vp), ("d_obj", c_vp), ("d_probe", c_vp), ("probe_k", c_vp), ("gom", c_vp),
         ("red", c_vp), ("errors", c_vp), ("aux", c_vp), ("finite_flag", c_vp),
         ("psi_slots", c_i32), ("pad_", c_i32), ("grad_obj", c_vp), ("grad_probe", c_vp),
+        ("chi_slices", c_vp),
     ]
 
 
@@ -56,7 +57,11 @@ SIGNATURES = {
     "phx_forward_slice": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_farfield": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_lsqml_backward": (c_int, [_P, c_int, c_int, c_int, c_vp]),
+    "phx_group_forward": (c_int, [_P, c_vp]),
+    "phx_farfield_update": (c_int, [_P, c_int, c_vp]),
+    "phx_lsqml_backward_sweep": (c_int, [_P, c_int, c_int, c_int, c_vp]),
     "phx_lsqml_object_update": (c_int, [_P, c_int, c_vp]),
+    "phx_lsqml_object_update_slices": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_probe_spectrum_reduce": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply": (c_int, [_P, c_vp]),
     "phx_epie_backward": (c_int, [_P, c_int, c_int, c_int, c_vp]),

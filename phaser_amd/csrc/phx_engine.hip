@@ -294,10 +294,16 @@ __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int cal
   }
 }
 
+// chi entering the step of slice i, and the buffer the step writes chi_{i-1} to.  With
+// chi_slices every slice keeps its chi (chi_{S-1} from the far field stays in chi_a);
+// otherwise chi ping-pongs between chi_a and chi_b.
 __device__ __forceinline__ float2* chi_in_buf(const phx_group_desc& d, int i) {
+  if (d.chi_slices && i < d.n_slices - 1)
+    return (float2*)d.chi_slices + (size_t)i * d.bmax * d.n_modes * d.n * d.n;
   return (float2*)(((d.n_slices - 1 - i) % 2 == 0) ? d.chi_a : d.chi_b);
 }
 __device__ __forceinline__ float2* chi_out_buf(const phx_group_desc& d, int i) {
+  if (d.chi_slices) return (float2*)d.chi_slices + (size_t)(i - 1) * d.bmax * d.n_modes * d.n * d.n;
   return (float2*)(((d.n_slices - 1 - i) % 2 == 0) ? d.chi_b : d.chi_a);
 }
 
@@ -387,68 +393,286 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Fused group sweeps.  One workgroup carries one (position slot b, mode m) plane through
+// every slice: the wave stays in registers between slice steps, so per step it reads
+// only the slice's object cutout and propagator (both L2-resident across the group's
+// planes) and writes the stored wave (or chi).  This replaces S+1 per-slice launches and
+// the per-slice HBM reload of the wave.
+
+// Forward model of a group (simulation.py:241-285; solvers.py:202-236, 431-457):
+//   psi_0 = ifft2(fft2(P_m) filt_b), psi_{i+1} = ifft2(fft2(psi_i O_i) p_i),
+//   farfield = FFT(psi_{S-1} O_{S-1}) (unnormalised), intensity += |farfield|^2 / n^2,
+// storing every psi_i and the norm sum_m |psi_0|^2 of each slot.
+template <int N, int T>
+__global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr int E = Plane<N>::E;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, S = d.n_slices;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+  const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
+  float nrm = 0.f;  // per-lane FP32 partial of sum |psi_0|^2, FP64 block sum
+  auto no_pin = [] {};
+
+  float2 x[E];
+  pl.cols_apply(x, [&](int py, int px, float2) {
+    return cmul(Pk[py * N + px], shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]));
+  }, no_pin);
+  pl.inverse_regs(x);
+  for (int i = 0;; ++i) {
+    pl.refresh();
+    // x: n^2 psi_i in row phase (natural order)
+    float2* psi_i = (float2*)d.psi + psi_off(d, i, b, m);
+    const float2* obj_i = (const float2*)d.object + i * HW;
+    const bool first = i == 0;
+    pl.rows_apply(x, [&](int yy, int xx, float2 v) {
+      v = cscale(v, inv_nn);
+      psi_i[yy * N + xx] = v;
+      if (first) nrm += abs2f(v);
+      return cmul(v, obj_at(obj_i, c, yy, xx));
+    }, [&] { __asm__ volatile("" : "+v"(nrm)); });
+    if (i == S - 1) break;
+    const float2* prop = (const float2*)d.props + (size_t)i * NN;
+    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); });
+  }
+  pl.forward_regs(x);
+  float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
+  float* I = d.intensity + (size_t)b * NN;
+  const bool single = M == 1;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int e = (4 * k + pl.q) * N + pl.rc;
+    F[e] = x[k];
+    const float a = abs2f(x[k]) * inv_nn;
+    if (single) I[e] = a;
+    else atomic_add_f32(&I[e], a);
+  }
+  const double tot = block_sum_f64(nrm, red_lds_);
+  if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(S, d.bmax, M).norm + b], tot);
+}
+
+// LSQ-ML backward sweep of a group (solvers.py:231-271): the modulus update of the far
+// field (noise_models.py:47-63, error of mode 0 when calc_error), then for i = S-1 .. 0
+// with chi_i in registers: store chi_i (for the object update), the FP64 alpha_O sums
+// of slice i, dP = chi_i conj(O_i), and either chi_{i-1} = ifft2(fft2(dP) conj(p_{i-1}))
+// or, at slice 0 with a probe update, the slot's probe spectrum fft2(dP) conj(filt_b)
+// into the farfield buffer with the alpha_P sums.  Object and probe updates follow in
+// their own launches; the sweep reads only the pre-update cutouts (solvers.py:246-260).
+template <int N, int T, bool UPD_OBJ, bool PROBE>
+__global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, int calc_error) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr int E = Plane<N>::E;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, S = d.n_slices, bmax = d.bmax;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const RedLayout L = red_layout(S, bmax, M);
+
+  // far field: chi_hat = (sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask F
+  const float2* F = (const float2*)d.farfield + ((size_t)b * M + m) * NN;
+  const float* I = d.intensity + (size_t)b * NN;
+  const float* D = d.patterns_s + (size_t)p * NN;
+  const float* mask = d.mask_s;
+  const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
+  float e2 = 0.f;
+  float2 x[E];
+  pl.cols_apply(x, [&](int py, int px, float2) {
+    const int e = py * N + px;
+    const float Iv = I[e], Dv = D[e], mk = mask[e];
+    const float2 Fv = F[e];
+    const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
+    const float df = Iv - Dv;
+    const float t = df * df;  // unmasked (solvers.py:237), NaN-skipping (nansum)
+    e2 += isnan(t) ? 0.f : t;
+    return cscale(Fv, (sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f) * mk);
+  }, [&] { __asm__ volatile("" : "+v"(e2)); });
+  if (calc_error && m == 0) {
+    const double tot = block_sum_f64(e2, red_lds_);
+    if (threadIdx.x == 0) d.errors[b] = (float)sqrt(tot);
+  }
+  pl.inverse_regs(x);
+
+  // one slice step on chi_i (row phase, unscaled): store it, the alpha_O sums, and
+  // x <- dP = chi_i conj(O_i); at slice 0 also the alpha_P sums
+  auto slice_step = [&](int i, auto probe_tag) {
+    constexpr bool PS = decltype(probe_tag)::value;
+    float2* chi_i = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
+    const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
+    const float2* obj_i = (const float2*)d.object + i * HW;
+    float numO = 0.f, denO = 0.f, numP = 0.f, denP = 0.f;
+    pl.rows_apply(x, [&](int yy, int xx, float2 v) {
+      const float2 cv = cscale(v, inv_nn);
+      chi_i[yy * N + xx] = cv;
+      if (UPD_OBJ) {
+        const float2 ps = psi_i[yy * N + xx];
+        const float2 t = cmul(cmulc(cv, ps), ps);  // (chi conj(psi)) psi
+        numO += cv.x * t.x + cv.y * t.y;
+        denO += abs2f(t);
+      }
+      const float2 o = obj_at(obj_i, c, yy, xx);
+      const float2 dP = cmulc(cv, o);  // chi conj(O)
+      if (PS) {
+        const float2 t2 = cmul(dP, o);
+        numP += cv.x * t2.x + cv.y * t2.y;
+        denP += abs2f(t2);
+      }
+      return dP;
+    }, [&] {
+      if (UPD_OBJ) __asm__ volatile("" : "+v"(numO), "+v"(denO));
+      if (PS) __asm__ volatile("" : "+v"(numP), "+v"(denP));
+    });
+    if (UPD_OBJ) {
+      const double a = block_sum_f64(numO, red_lds_);
+      const double bb = block_sum_f64(denO, red_lds_);
+      if (threadIdx.x == 0) {
+        atomic_add_f64(&d.red[L.num_obj + i * bmax + b], a);
+        atomic_add_f64(&d.red[L.den_obj + i], bb);
+      }
+    }
+    if (PS) {
+      const double a = block_sum_f64(numP, red_lds_);
+      const double bb = block_sum_f64(denP, red_lds_);
+      if (threadIdx.x == 0) {
+        d.red[L.numP + b * M + m] = a;
+        atomic_add_f64(&d.red[L.denP], bb);
+      }
+    }
+  };
+  for (int i = S - 1; i > 0; --i) {
+    pl.refresh();
+    slice_step(i, std::false_type{});
+    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); });
+  }
+  pl.refresh();
+  slice_step(0, std::integral_constant<bool, PROBE>{});
+  if (PROBE) {
+    // per-plane fft2(dP) conj(filt_b) into this plane's (already consumed) farfield slot
+    const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+    float2* Fp = (float2*)d.farfield + ((size_t)b * M + m) * NN;
+    pl.forward_regs(x);
+    const int px = pl.rc;
+    const float kx = d.kx_s[px];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int py = 4 * k + pl.q;
+      Fp[py * N + px] = cmulc(x[k], shift_filter(sy, sx, d.ky_s[py], kx));
+      if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // a few phase factors at a time
+    }
+  }
+}
+
 // LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
 //   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
 //   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
-// One workgroup owns a 16x16 object tile (one pixel per thread) and gathers both sums
-// from the group's chi_i and psi_i planes, positions in group order and modes in order:
-// no atomics, and the result does not depend on scheduling.  Pixels no cutout covers are
+// One workgroup owns a 16x16 object tile of slice first + blockIdx.z (one pixel per
+// thread).  Wave 0 lists the group's positions whose cutout meets the tile, in group
+// order (ballot + prefix count); then every thread gathers both sums from those
+// positions' chi_i and psi_i planes, positions in group order and modes in order: no
+// atomics, and the result does not depend on scheduling.  Pixels no cutout covers are
 // left untouched (their reference update is exactly zero).
 constexpr int UPD_TILE = 16;
-__global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int i) {
+constexpr int UPD_LIST = 256;  // positions listed per pass
+__global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int first) {
+  const int i = first + (int)blockIdx.z;
   __shared__ float sA;
+  __shared__ int s_b[UPD_LIST], s_y0[UPD_LIST], s_x0[UPD_LIST];
+  __shared__ int s_cnt;
   const RedLayout L = red_layout(d.n_slices, d.bmax, d.n_modes);
   const int gs = *d.group_size;
-  if (threadIdx.x < 64) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave == 0) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < gs; b += 64) s += d.red[L.num_obj + i * d.bmax + b];
+    for (int b = lane; b < gs; b += 64) s += d.red[L.num_obj + i * d.bmax + b];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
       const double den = d.red[L.den_obj + i] + (double)d.params[P_GAMMA];
       sA = (float)((double)d.params[P_BETA_OBJ] * (s / den));
     }
   }
-  __syncthreads();
-  const float A = sA, reg = d.params[P_REG_OBJ];
+  const float reg = d.params[P_REG_OBJ];
   const int n = d.n, NN = n * n, M = d.n_modes, H = d.obj_h, W = d.obj_w;
   const int ty0 = blockIdx.y * UPD_TILE, tx0 = blockIdx.x * UPD_TILE;
   const int y = ty0 + threadIdx.x / UPD_TILE, x = tx0 + threadIdx.x % UPD_TILE;
   const float2* chi = chi_in_buf(d, i);
   const float2* psi = (const float2*)d.psi;
-  const size_t mstride_psi = (size_t)NN;  // consecutive modes of one slot are adjacent planes
   float2 dO = make_float2(0.f, 0.f);
   float g = 0.f;
-  for (int b = 0; b < gs; ++b) {
-    const int p = d.group_pos[b];
-    const int y0 = d.start[2 * p], x0 = d.start[2 * p + 1];
-    if (y0 >= ty0 + UPD_TILE || y0 + n <= ty0 || x0 >= tx0 + UPD_TILE || x0 + n <= tx0) continue;
-    const int ly = y - y0, lx = x - x0;
-    if (ly < 0 || ly >= n || lx < 0 || lx >= n) continue;
-    const size_t off = (size_t)ly * n + lx;
-    const float2* cb = chi + (size_t)b * M * NN + off;
-    const float2* pb = psi + psi_off(d, i, b, 0) + off;
-    int m = 0;
-    for (; m + 4 <= M; m += 4) {
-      const float2 c0 = cb[(m + 0) * mstride_psi], c1 = cb[(m + 1) * mstride_psi];
-      const float2 c2 = cb[(m + 2) * mstride_psi], c3 = cb[(m + 3) * mstride_psi];
-      const float2 p0 = pb[(m + 0) * mstride_psi], p1 = pb[(m + 1) * mstride_psi];
-      const float2 p2 = pb[(m + 2) * mstride_psi], p3 = pb[(m + 3) * mstride_psi];
-      dO = cadd(dO, cmulc(c0, p0));
-      dO = cadd(dO, cmulc(c1, p1));
-      dO = cadd(dO, cmulc(c2, p2));
-      dO = cadd(dO, cmulc(c3, p3));
-      g += abs2f(p0);
-      g += abs2f(p1);
-      g += abs2f(p2);
-      g += abs2f(p3);
+  for (int c0 = 0; c0 < gs; c0 += UPD_LIST) {
+    const int c1 = min(gs, c0 + UPD_LIST);
+    __syncthreads();  // the previous list is consumed (and sA is visible)
+    if (wave == 0) {
+      int cnt = 0;
+      for (int b0 = c0; b0 < c1; b0 += 64) {
+        const int b = b0 + lane;
+        bool ov = false;
+        int y0 = 0, x0 = 0;
+        if (b < c1) {
+          const int p = d.group_pos[b];
+          y0 = d.start[2 * p];
+          x0 = d.start[2 * p + 1];
+          ov = !(y0 >= ty0 + UPD_TILE || y0 + n <= ty0 || x0 >= tx0 + UPD_TILE || x0 + n <= tx0);
+        }
+        const uint64_t msk = __ballot(ov);
+        if (ov) {
+          const int r = cnt + __popcll(msk & ((1ull << lane) - 1ull));
+          s_b[r] = b;
+          s_y0[r] = y0;
+          s_x0[r] = x0;
+        }
+        cnt += __popcll(msk);
+      }
+      if (lane == 0) s_cnt = cnt;
     }
-    for (; m < M; ++m) {
-      const float2 cv = cb[m * mstride_psi], ps = pb[m * mstride_psi];
-      dO = cadd(dO, cmulc(cv, ps));
-      g += abs2f(ps);
+    __syncthreads();
+    const int cnt = s_cnt;
+    for (int t = 0; t < cnt; ++t) {
+      const int b = s_b[t];
+      const int ly = y - s_y0[t], lx = x - s_x0[t];
+      // branch-free: clamped (valid) address, contribution weighted by the in-cutout test
+      const float w = ((unsigned)ly < (unsigned)n && (unsigned)lx < (unsigned)n) ? 1.f : 0.f;
+      const size_t off = (size_t)min(max(ly, 0), n - 1) * n + min(max(lx, 0), n - 1);
+      const float2* cb = chi + (size_t)b * M * NN + off;
+      const float2* pb = psi + psi_off(d, i, b, 0) + off;
+      float2 dOb = make_float2(0.f, 0.f);
+      float gb = 0.f;
+      int m = 0;
+      for (; m + 4 <= M; m += 4) {
+        const float2 c0v = cb[(m + 0) * NN], c1v = cb[(m + 1) * NN];
+        const float2 c2v = cb[(m + 2) * NN], c3v = cb[(m + 3) * NN];
+        const float2 p0 = pb[(m + 0) * NN], p1 = pb[(m + 1) * NN];
+        const float2 p2 = pb[(m + 2) * NN], p3 = pb[(m + 3) * NN];
+        dOb = cadd(dOb, cmulc(c0v, p0));
+        dOb = cadd(dOb, cmulc(c1v, p1));
+        dOb = cadd(dOb, cmulc(c2v, p2));
+        dOb = cadd(dOb, cmulc(c3v, p3));
+        gb += abs2f(p0);
+        gb += abs2f(p1);
+        gb += abs2f(p2);
+        gb += abs2f(p3);
+      }
+      for (; m < M; ++m) {
+        const float2 cv = cb[m * NN], ps = pb[m * NN];
+        dOb = cadd(dOb, cmulc(cv, ps));
+        gb += abs2f(ps);
+      }
+      dO = cadd(dO, cscale(dOb, w));
+      g += gb * w;
     }
   }
   if (y >= H || x >= W || (g == 0.f && dO.x == 0.f && dO.y == 0.f)) return;
+  const float A = sA;
   float2* obj_i = (float2*)d.object + (size_t)i * H * W;
   const float2 q = cscale(dO, 1.0f / (g + reg));
   const float2 u = cscale(cscale(cscale(q, A), g), 1.0f / (g + 1e-16f));
@@ -653,7 +877,7 @@ __global__ void __launch_bounds__(T) position_gradient_k(phx_group_desc d, int d
   const RedLayout L = red_layout(d.n_slices, bmax, M);
   const Cut c = cut_of(d, d.group_pos[b]);
   const float2* psi0 = (const float2*)d.psi + psi_off(d, 0, b, m);
-  const float2* chi = (const float2*)((d.n_slices % 2 == 1) ? d.chi_a : d.chi_b) + ((size_t)b * M + m) * NN;
+  const float2* chi = chi_in_buf(d, 0) + ((size_t)b * M + m) * NN;  // the final chi (solvers.py:275-288)
   const float2* obj0 = (const float2*)d.object;
   const float scale = (float)(1.0 / ((double)NN * d.red[L.norm + b]));
   float num = 0.f, den = 0.f;  // per-lane FP32 partials, FP64 block sums
@@ -979,14 +1203,74 @@ int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, in
   return PHX_OK;
 }
 
-int phx_lsqml_object_update(const phx_group_desc* d, int slice, hipStream_t stream) {
+int phx_group_forward(const phx_group_desc* d, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || !d->finite_flag) return PHX_EINVAL;
-  const dim3 grid((unsigned)((d->obj_w + UPD_TILE - 1) / UPD_TILE), (unsigned)((d->obj_h + UPD_TILE - 1) / UPD_TILE));
-  hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, slice);
+  if (!d->probe_k || !d->farfield || !d->intensity || (d->n_slices > 1 && !d->props)) return PHX_EINVAL;
+  if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // stores every slice
+#define L(N, T) hipLaunchKernelGGL((group_forward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d)
+  PHX_DISPATCH(d->n, L);
+#undef L
   PHX_CHECK_LAUNCH();
   return PHX_OK;
+}
+
+int phx_farfield_update(const phx_group_desc* d, int calc_error, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a || (calc_error && !d->errors))
+    return PHX_EINVAL;
+#define L(N, T) \
+  hipLaunchKernelGGL((farfield_update_k<N, T, FM_MODULUS>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_backward_sweep(const phx_group_desc* d, int update_object, int update_probe, int calc_error,
+                             hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a || (calc_error && !d->errors) ||
+      (d->n_slices > 1 && (!d->props || !d->chi_slices)))
+    return PHX_EINVAL;
+  if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;
+#define K(N, T, UO, PR) \
+  hipLaunchKernelGGL((lsqml_backward_fused_k<N, T, UO, PR>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error)
+#define L(N, T)                       \
+  do {                                \
+    if (update_object && update_probe) \
+      K(N, T, true, true);            \
+    else if (update_object)           \
+      K(N, T, true, false);           \
+    else if (update_probe)            \
+      K(N, T, false, true);           \
+    else                              \
+      K(N, T, false, false);          \
+  } while (0)
+  PHX_DISPATCH(d->n, L);
+#undef L
+#undef K
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (first < 0 || count < 1 || first + count > d->n_slices || !d->chi_a || !d->chi_b || !d->finite_flag ||
+      (count > 1 && !d->chi_slices))
+    return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->obj_w + UPD_TILE - 1) / UPD_TILE), (unsigned)((d->obj_h + UPD_TILE - 1) / UPD_TILE),
+                  (unsigned)count);
+  hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, first);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_object_update(const phx_group_desc* d, int slice, hipStream_t stream) {
+  return phx_lsqml_object_update_slices(d, slice, 1, stream);
 }
 
 int phx_probe_spectrum_reduce(const phx_group_desc* d, hipStream_t stream) {

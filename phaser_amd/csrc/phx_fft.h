@@ -129,6 +129,11 @@ struct Plane {
     q = l & 3;
   }
 
+  // Makes rc and q opaque for one loop iteration: called at the top of a loop over
+  // slices, it keeps the compiler from hoisting every point's address arithmetic out of
+  // the loop (dozens of registers live across all transforms).
+  __device__ __forceinline__ void refresh() { __asm__ volatile("" : "+v"(rc), "+v"(q)); }
+
   // ----- one 1-D transform along the quad (registers + DPP only) -----
   // The DFT-4 across the quad is two radix-2 exchanges (partner q^2, then q^1), each
   // written as one FMA per component with a per-lane sign (+-1 is exact), and the one
@@ -202,6 +207,9 @@ struct Plane {
       x[j] = src(rc, 4 * j + q);
       if ((j % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || j == E - 1) {
         pin();
+        // the group's global loads first, then its arithmetic
+        __builtin_amdgcn_sched_group_barrier(0x0020, 4 * PHX_LOAD_PACE, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
@@ -209,6 +217,34 @@ struct Plane {
   template <class Src>
   __device__ __forceinline__ void load_rows(Src src, float2 (&x)[E]) {
     load_rows(src, x, [] {});
+  }
+  // x[j] = f(y, x, x[j]) over the row-phase points (natural order), in load groups
+  template <class F, class Pin>
+  __device__ __forceinline__ void rows_apply(float2 (&x)[E], F f, Pin pin) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      x[j] = f(rc, 4 * j + q, x[j]);
+      if ((j % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || j == E - 1) {
+        pin();
+        __builtin_amdgcn_sched_group_barrier(0x0020, 4 * PHX_LOAD_PACE, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  // x[k] = f(py, px, x[k]) over the column-phase points (scrambled spectrum), in load groups
+  template <class F, class Pin>
+  __device__ __forceinline__ void cols_apply(float2 (&x)[E], F f, Pin pin) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      x[k] = f(4 * k + q, rc, x[k]);
+      if ((k % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || k == E - 1) {
+        pin();
+        __builtin_amdgcn_sched_group_barrier(0x0020, 4 * PHX_LOAD_PACE, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
   }
   // registers (row phase, natural) -> sink dst(y, x, v)
   template <class Dst>
@@ -250,6 +286,8 @@ struct Plane {
       x[k] = src(4 * k + q, rc);
       if ((k % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || k == E - 1) {
         pin();
+        __builtin_amdgcn_sched_group_barrier(0x0020, 4 * PHX_LOAD_PACE, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
@@ -264,8 +302,13 @@ struct Plane {
   template <class Mul>
   __device__ __forceinline__ void convolve_regs(float2 (&x)[E], Mul mul) {
     forward_regs(x);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < E; ++k) x[k] = mul(4 * k + q, rc, x[k]);
+    // the multiplier's loads (a propagator: L2-resident) all issued before the products
+    __builtin_amdgcn_sched_group_barrier(0x0020, 2 * E, 0);
+    __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
+    __builtin_amdgcn_sched_barrier(0);
     inverse_regs(x);
   }
   template <class Src, class Mul, class Dst, class Pin>

@@ -59,7 +59,7 @@ class GroupWorkspace:
     """Buffers and descriptors for one engine run over a fixed set of positions."""
 
     def __init__(self, state, patterns, pattern_mask, props, *, noise_offset, noise_eps, n_groups, bmax,
-                 noise_var=1.1, psi_slots=0, gradients=False, device='cuda'):
+                 noise_var=1.1, psi_slots=0, gradients=False, keep_chi=False, fused=True, device='cuda'):
         torch = _require_torch_gpu()
         _lib.load()
         self.torch = torch
@@ -125,7 +125,12 @@ class GroupWorkspace:
         self.pos_update = torch.zeros((self.P, 2), dtype=f32, device=dev)
         self.grad_obj = torch.zeros((S, H, W), dtype=c64, device=dev) if gradients else None
         self.grad_probe = torch.zeros((M, NN), dtype=c64, device=dev) if gradients else None
+        # LSQ-ML keeps every slice's chi so one launch updates all slices of a group
+        self.chi_slices = (torch.empty((S - 1, B, M, NN), dtype=c64, device=dev)
+                           if keep_chi and S > 1 else None)
 
+        # whole-sweep kernels (one launch per sweep) instead of one launch per slice
+        self.fused = bool(fused)
         self.object = obj
         self.probe = probe
         self.descs = [self._make_desc(g) for g in range(self.G)]
@@ -157,6 +162,7 @@ class GroupWorkspace:
         d.psi_slots = self.psi_slots
         d.grad_obj = None if self.grad_obj is None else self.grad_obj.data_ptr()
         d.grad_probe = None if self.grad_probe is None else self.grad_probe.data_ptr()
+        d.chi_slices = None if self.chi_slices is None else self.chi_slices.data_ptr()
         return d
 
     # ------------------------------------------------------------------ host tables
@@ -209,11 +215,18 @@ class GroupWorkspace:
             c("phx_forward_slice", d, i, 0, st)
         c("phx_dry_run_stats", d, st)
 
-    def launch_forward(self, g, calc_error, illum=True):
+    def launch_forward(self, g, calc_error, illum=True, modulus=True):
+        """Forward model of group slot g up to the exit spectrum and intensity, then (with
+        `modulus`) the far-field modulus update into chi_a."""
         d, st = self.desc_refs[g], stream_handle()
         c = self.call
         c("phx_group_begin", d, st)
         c("phx_probe_spectra", d, st)
+        if self.fused and not illum and self.psi_slots >= self.S:
+            c("phx_group_forward", d, st)   # all slices in one launch
+            if modulus:
+                c("phx_farfield_update", d, int(calc_error), st)
+            return
         c("phx_shift_probes", d, int(illum), int(self.S > 1), st)
         for i in range(1, self.S - 1):
             c("phx_forward_slice", d, i, int(illum), st)
@@ -223,18 +236,38 @@ class GroupWorkspace:
         """lsqml_run (solvers.py:171-290) for group slot g."""
         d, st = self.desc_refs[g], stream_handle()
         c = self.call
+        if self.fused and (self.chi_slices is not None or self.S == 1):
+            # forward and backward sweeps in one launch each; every update follows the
+            # backward sweep, which reads only pre-update cutouts (solvers.py:246-260)
+            self.launch_forward(g, calc_error, illum=False, modulus=False)
+            if update_probe:
+                c("phx_group_obj_mag", d, st)
+            c("phx_lsqml_backward_sweep", d, int(update_object), int(update_probe), int(calc_error), st)
+            if update_position:
+                self._launch_position(d, st)
+            if update_object:
+                c("phx_lsqml_object_update_slices", d, 0, self.S, st)
+            if update_probe:
+                c("phx_probe_spectrum_reduce", d, st)
+                c("phx_lsqml_probe_apply", d, st)
+            return
         # illumination sums are gathered by the object update, not scattered here
         self.launch_forward(g, calc_error, illum=False)
         if update_probe:
             c("phx_group_obj_mag", d, st)
+        deferred = self.chi_slices is not None or self.S == 1
         for i in range(self.S - 1, -1, -1):
             c("phx_lsqml_backward", d, i, int(update_object), int(update_probe), st)
             if i == 0 and update_position:
                 # the position step uses the group's pre-update slice-0 cutout and the
                 # final chi (solvers.py:275-288), so it runs before slice 0 is updated
                 self._launch_position(d, st)
-            if update_object:
+            if update_object and not deferred:
                 c("phx_lsqml_object_update", d, i, st)
+        if update_object and deferred:
+            # the backward sweep reads only pre-update cutouts (solvers.py:246-260), so
+            # every slice's update can follow it, all in one launch
+            c("phx_lsqml_object_update_slices", d, 0, self.S, st)
         if update_probe:
             c("phx_probe_spectrum_reduce", d, st)
             c("phx_lsqml_probe_apply", d, st)

@@ -167,12 +167,17 @@ class EPIESolver(_SolverBase):
 SOLVER_CLASSES = {'lsqml': LSQMLSolver, 'epie': EPIESolver}
 
 
-def run_engine(args, props, *, use_graphs=True):
+def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     """Conventional engine (phaser/engines/conventional/run.py:15-122).
 
     args: EngineArgs mapping {data: Patterns, state: ReconsState, dtype, xp, recons_name,
     observer, seed}.  state.object.data / state.probe.data may be numpy arrays or torch
-    tensors; the returned state holds torch tensors on the GPU."""
+    tensors; the returned state holds torch tensors on the GPU.
+
+    keep_chi (LSQ-ML; default on): keep every slice's back-propagated chi, (S-1) B M n^2
+    complex64, so a group's object updates run as one launch after the backward sweep
+    instead of one per slice inside it.  fused: whole-sweep kernels (one launch for the
+    forward model, one for the LSQ-ML backward sweep) instead of one launch per slice."""
     dtype = args.get('dtype', numpy.float32)
     if numpy.dtype(dtype) != numpy.float32:
         raise NotImplementedError("the MI355X backend runs float32 / complex64 plans")
@@ -211,7 +216,9 @@ def run_engine(args, props, *, use_graphs=True):
     n_pos = int(numpy.prod(state.scan.shape[:-1]))
     bmax = int(numpy.ceil(n_pos / len(groups)))
     ws = GroupWorkspace(state, patterns, pattern_mask, propagators, noise_offset=noise.offset, noise_eps=noise.eps,
-                        n_groups=len(groups), bmax=bmax)
+                        n_groups=len(groups), bmax=bmax,
+                        keep_chi=(props.solver.name == 'lsqml') if keep_chi is None else bool(keep_chi),
+                        fused=fused)
     ws.set_positions(state.object.sampling, state.scan)
     sim = SimulationState(state=state, noise_model=noise, workspace=ws)
     sim = solver.init(sim)

@@ -142,3 +142,30 @@ def test_lsqml_position_correction():
           flush=True)
     assert numpy.abs(d_o).max() > 0
     assert e_g64 <= 3 * e_o64 + 1e-5
+
+
+@pytest.mark.parametrize('solver', ['lsqml', 'epie'])
+def test_fused_sweeps_and_deferred_updates_equal_per_slice_launches(solver):
+    """Whole-sweep kernels (one launch for the forward model, one for the LSQ-ML backward
+    sweep, every object update after the sweep) give the per-slice launch sequence's
+    result: the sweep reads only pre-update cutouts (solvers.py:246-260)."""
+    from phaser_amd.engines.conventional import run_engine
+    from phaser_amd.plan import ConventionalEnginePlan, MomentumPositionSolverPlan
+    _, state, pats = _problem(n=32, n_modes=2, n_slices=3, scan_shape=(6, 6), scan_step_px=3.0, jitter_px=0.5,
+                              seed=5)
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver=solver, niter=2, grouping=12,
+                                  position_solver=MomentumPositionSolverPlan(step_size=1e-2),
+                                  update_positions=True)
+    variants = [dict(keep_chi=False, fused=False), dict(keep_chi=True, fused=False), dict(keep_chi=True, fused=True)]
+    out = []
+    for kw in variants:
+        args = {'data': pats, 'state': state.copy(), 'dtype': numpy.float32, 'seed': 3, 'observer': None,
+                'recons_name': 't'}
+        out.append(run_engine(args, plan, use_graphs=False, **kw).to_numpy())
+    ref = out[0]
+    for kw, o in zip(variants[1:], out[1:]):
+        eo, ep, es = (rel_err(o.object.data, ref.object.data), rel_err(o.probe.data, ref.probe.data),
+                      rel_err(o.scan, ref.scan))
+        print(f"{kw}: object {eo:.2e} probe {ep:.2e} scan {es:.2e}", flush=True)
+        assert eo < 1e-6 and ep < 1e-6 and es < 1e-6
+        numpy.testing.assert_allclose(o.progress.detector_errors, ref.progress.detector_errors, rtol=1e-6)
