@@ -131,6 +131,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-groups", type=int, default=8)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-initializer time-to-NRMSE comparison")
+    ap.add_argument("--learn-objects", type=int, default=48)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a device buffer of this many MiB once (a copy kernel of "
                          "known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -220,6 +223,22 @@ def main():
                "sample": f"{npat} patterns x 1 LSQ-ML group step ({args.cpu_groups} group(s) of this workload) "
                          f"with the FP32 numpy oracle on the host, {dt:.1f} s; host has {os.cpu_count()} cores"}
 
+    ttn = None
+    if not args.no_learned:
+        # time to object NRMSE <= 0.05 on this workload: learned init + adjoint gradient
+        # engine vs plain LSQ-ML from the flat start (network trained first, off the clock)
+        from dataclasses import replace
+        from phaser_amd import learned
+        log(f"learned initializer: training on {args.learn_objects} plain-LSQ-ML reconstructions")
+        base = replace(cfg, n_slices=3, scan_shape=(16, 16), jitter_px=0.5)
+        mlp, info = learned.train_on_reconstructions(args.learn_objects, base=base, niter=40, epochs=30, lr=1e-4)
+        log(f"trained: {json.dumps(info)}")
+        ttn = learned.time_to_nrmse(cfg, mlp, threshold=0.05, max_iter=60, data=(state0, pats, truth), log=log)
+        for k in ('plain_lsqml', 'learned_gradient', 'flat_gradient'):
+            h = ttn[k].pop('history')
+            ttn[k]['nrmse_by_iteration'] = [e for _t, _i, e in h]
+        ttn['training'] = info
+
     line = {
         "metric": METRIC, "value": value, "unit": "patterns·iter/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
@@ -233,6 +252,7 @@ def main():
                      "whole_iteration_alg_GBs": whole, "whole_iteration_frac": whole / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "detector_error_first_last": [errs[0], errs[-1]],
+        "time_to_nrmse": ttn,
     }
     print(json.dumps(line), flush=True)
     if args.calib_copy_mb:

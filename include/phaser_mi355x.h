@@ -262,6 +262,47 @@ int phx_exit_intensity(const phx_group_desc* d, float* out, hipStream_t stream);
  * both phx_position_gradient calls; pos_update is (P, 2) float32. */
 int phx_position_finish(const phx_group_desc* d, void* pos_update, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------------
+ * Learned initializer (no counterpart in the reference: an MLP that maps binned
+ * diffraction patterns to object-patch phase and amplitude, trained on the engines' own
+ * reconstructions; its guesses are stitched into the starting object).
+ * BF16 operands (raw 16-bit bfloat16), FP32 accumulation.
+ * --------------------------------------------------------------------------------- */
+/* C[M, N] (FP32 and/or BF16 Cb, leading dimension ldc) = op(A) op(B), op(A) = A (M x K,
+ * row-major, lda) or, with trans_a, the transpose of a K x M array; op(B) = B (K x N) or,
+ * with trans_b, the transpose of an N x K array.  Epilogue in order: + C when
+ * accumulate, + bias[n], max(., 0) when relu, zeroed where mask (BF16, ldm) is <= 0
+ * (the ReLU derivative of a stored activation).  M, N multiples of 128; K of 32; lda, ldb
+ * multiples of 8. */
+int phx_gemm_bf16(const void* A, int64_t lda, int trans_a, const void* B, int64_t ldb, int trans_b, int64_t M,
+                  int64_t N, int64_t K, float* C, void* Cb, int64_t ldc, const float* bias, int relu, const void* mask,
+                  int64_t ldm, int accumulate, hipStream_t stream);
+/* out[c] = sum_r in[r * ld + c], rows summed in order (FP32). */
+int phx_colsum_f32(const float* in, int64_t rows, int64_t cols, int64_t ld, float* out, hipStream_t stream);
+/* Adam on FP32 master weights w (m, v moments; c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)); the BF16
+ * copy wb (may be NULL) is refreshed.  g += weight_decay * w first. */
+int phx_adam_f32_bf16(float* w, void* wb, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                      float eps, float c1, float c2, float weight_decay, hipStream_t stream);
+int phx_f32_to_bf16(const float* in, void* out, int64_t n, hipStream_t stream);
+/* d = scale (pred - target) (FP32 d and/or BF16 db may be NULL); *loss += sum (pred -
+ * target)^2 in FP64 when loss != NULL. */
+int phx_mse_grad(const float* pred, const float* target, int64_t n, float scale, float* d, void* db, double* loss,
+                 hipStream_t stream);
+/* Network input: out[r] = log(1 + bin2x2(pattern[index[r]]) * inv_scale), (n/2)^2 BF16
+ * values per row; patterns (P, n, n) FP32 corner-origin; index NULL means r. */
+int phx_pattern_features(const float* patterns, const int32_t* index, int64_t count, int64_t n, float inv_scale,
+                         void* out, hipStream_t stream);
+/* Training targets: out[r] = [phase - phase0, amplitude] of prod_s O_s over the ps x ps
+ * window starting at starts[r] = (y0, x0) (2 ps^2 FP32 per row). */
+int phx_object_patches(const void* obj, int64_t S, int64_t H, int64_t W, const int32_t* starts, int64_t count,
+                       int64_t ps, float phase0, float* out, hipStream_t stream);
+/* Stitch predicted rows [phase, amplitude] (row stride ldp) at their windows: per-pixel
+ * averages (acc_* zeroed by the caller), then obj[s] = a^(1/S) exp(i phi / S) for every
+ * slice, 1 where no window covers the pixel. */
+int phx_stitch_patches(const float* pred, int64_t ldp, const int32_t* starts, int64_t count, int64_t ps, int64_t S,
+                       int64_t H, int64_t W, float* acc_phase, float* acc_amp, float* acc_n, void* obj,
+                       hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,17 @@ SIGNATURES = {
     "phx_farfield_update": (c_int, [_P, c_int, c_vp]),
     "phx_lsqml_backward_sweep": (c_int, [_P, c_int, c_int, c_int, c_vp]),
     "phx_lsqml_object_update": (c_int, [_P, c_int, c_vp]),
+    "phx_gemm_bf16": (c_int, [c_vp, c_i64, c_int, c_vp, c_i64, c_int, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp,
+                              c_int, c_vp, c_i64, c_int, c_vp]),
+    "phx_colsum_f32": (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
+    "phx_adam_f32_bf16": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_f32,
+                                  c_f32, c_vp]),
+    "phx_f32_to_bf16": (c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "phx_mse_grad": (c_int, [c_vp, c_vp, c_i64, c_f32, c_vp, c_vp, c_vp, c_vp]),
+    "phx_pattern_features": (c_int, [c_vp, c_vp, c_i64, c_i64, c_f32, c_vp, c_vp]),
+    "phx_object_patches": (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_f32, c_vp, c_vp]),
+    "phx_stitch_patches": (c_int, [c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp,
+                                   c_vp]),
     "phx_lsqml_object_update_slices": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_probe_spectrum_reduce": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply": (c_int, [_P, c_vp]),

@@ -273,7 +273,9 @@ __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int cal
           const float si = sqrtf(Iv + off);
           const float sd = sqrtf(Dp + off) - si - eps;
           t = mk * sd * sd;
-          u = -4.0f * mk * sd / (var * si) * gscale;
+          // at exactly zero model intensity every Psi_m is 0 and |Psi|/sqrt(I) <= 1: the
+          // zero subgradient (autodiff of sqrt at 0 would give inf * 0 = NaN)
+          u = si > 0.f ? -4.0f * mk * sd / (var * si) * gscale : 0.f;
         } else {
           t = mk * (Iv + eps + Dp * (logf(Dp + eps) - logf(Iv + eps) - 1.0f));
           u = 2.0f * mk * (1.0f - Dp / (Iv + eps)) * gscale;
