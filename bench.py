@@ -227,13 +227,12 @@ def main():
     if not args.no_learned:
         # time to object NRMSE <= 0.05 on this workload: learned init + adjoint gradient
         # engine vs plain LSQ-ML from the flat start (network trained first, off the clock)
-        from dataclasses import replace
         from phaser_amd import learned
         log(f"learned initializer: training on {args.learn_objects} plain-LSQ-ML reconstructions")
-        base = replace(cfg, n_slices=3, scan_shape=(16, 16), jitter_px=0.5)
-        mlp, info = learned.train_on_reconstructions(args.learn_objects, base=base, niter=40, epochs=30, lr=1e-4)
+        mlp, info = learned.train_on_reconstructions(args.learn_objects, niter=40, epochs=30, lr=1e-4)
         log(f"trained: {json.dumps(info)}")
-        ttn = learned.time_to_nrmse(cfg, mlp, threshold=0.05, max_iter=60, data=(state0, pats, truth), log=log)
+        ttn = learned.time_to_nrmse(learned.evaluation_config(), mlp, threshold=0.05, max_iter=60, lr_object=30.0,
+                                    log=log)
         for k in ('plain_lsqml', 'learned_gradient', 'flat_gradient'):
             h = ttn[k].pop('history')
             ttn[k]['nrmse_by_iteration'] = [e for _t, _i, e in h]

@@ -10,13 +10,10 @@ pytestmark = pytest.mark.gpu
 
 def test_learned_init_gains_over_flat_start_and_reaches_threshold():
     from phaser_amd import learned
-    from phaser_amd.synthetic import SyntheticConfig
     mlp, info = learned.train_on_reconstructions(48, niter=40, epochs=30, lr=1e-4)
     print(info, flush=True)
     assert info['loss_first_last'][1] < info['loss_first_last'][0]
-    cfg = SyntheticConfig(n=128, n_modes=1, n_slices=4, scan_shape=(24, 24), scan_step_px=6.0, jitter_px=0.5,
-                          seed=7)
-    r = learned.time_to_nrmse(cfg, mlp, threshold=0.05, max_iter=40)
+    r = learned.time_to_nrmse(learned.evaluation_config(), mlp, threshold=0.05, max_iter=60, lr_object=30.0)
     print({k: v for k, v in r.items() if k not in ('plain_lsqml', 'learned_gradient', 'flat_gradient')}, flush=True)
     for k in ('plain_lsqml', 'learned_gradient', 'flat_gradient'):
         print(k, r[k]['seconds'], r[k]['iterations'], r[k]['final_nrmse'], flush=True)
