@@ -86,12 +86,14 @@ def time_kernels(ws, groups, torch, n_groups):
         durations.setdefault(name, []).append((e0, e1))
 
     ws.call = timed   # the engine's own launch sequence, each kernel bracketed by events
+    ws.concurrent = False   # one stream, so events bracket every kernel
     try:
         for g in range(n_groups):
             ws.launch_lsqml_group(g, update_object=True, update_probe=True, update_position=False,
                                   calc_error=False)
     finally:
         ws.call = _lib.call
+        ws.concurrent = True
     torch.cuda.synchronize()
     out = {}
     for k, evs in durations.items():

@@ -131,6 +131,9 @@ class GroupWorkspace:
 
         # whole-sweep kernels (one launch per sweep) instead of one launch per slice
         self.fused = bool(fused)
+        # a second stream (default priority) for the probe-side kernels of a group step
+        self.side_stream = torch.cuda.Stream(device=dev)
+        self.concurrent = True
         self.object = obj
         self.probe = probe
         self.descs = [self._make_desc(g) for g in range(self.G)]
@@ -238,18 +241,33 @@ class GroupWorkspace:
         c = self.call
         if self.fused and (self.chi_slices is not None or self.S == 1):
             # forward and backward sweeps in one launch each; every update follows the
-            # backward sweep, which reads only pre-update cutouts (solvers.py:246-260)
-            self.launch_forward(g, calc_error, illum=False, modulus=False)
+            # backward sweep, which reads only pre-update cutouts (solvers.py:246-260).
+            # The probe-side kernels (few workgroups each) run on a second stream beside
+            # the sweeps and the object update: group_obj_mag needs only the zeroed
+            # accumulators and the pre-update object, the probe step only the sweep's
+            # probe spectra and alpha_P sums; both streams join before the next group.
+            side = self.side_stream if self.concurrent else None
+            main = self.torch.cuda.current_stream()
+            sh = ctypes.c_void_p(side.cuda_stream) if side is not None else st
+            c("phx_group_begin", d, st)
+            c("phx_probe_spectra", d, st)
             if update_probe:
-                c("phx_group_obj_mag", d, st)
+                if side is not None:
+                    side.wait_stream(main)
+                c("phx_group_obj_mag", d, sh)
+            c("phx_group_forward", d, st)
             c("phx_lsqml_backward_sweep", d, int(update_object), int(update_probe), int(calc_error), st)
             if update_position:
                 self._launch_position(d, st)
+            if update_probe:
+                if side is not None:
+                    side.wait_stream(main)
+                c("phx_probe_spectrum_reduce", d, sh)
+                c("phx_lsqml_probe_apply", d, sh)
             if update_object:
                 c("phx_lsqml_object_update_slices", d, 0, self.S, st)
-            if update_probe:
-                c("phx_probe_spectrum_reduce", d, st)
-                c("phx_lsqml_probe_apply", d, st)
+            if update_probe and side is not None:
+                main.wait_stream(side)
             return
         # illumination sums are gathered by the object update, not scattered here
         self.launch_forward(g, calc_error, illum=False)
