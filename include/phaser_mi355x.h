@@ -183,7 +183,7 @@ int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, in
 /* LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
  * O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16) with
  * dO = sum_{b,m} scatter(chi_i conj(psi_i)) and gpm = sum_{b,m} scatter(|psi_i|^2), both
- * gathered per 16x16 object tile in position order (no atomics, deterministic).
+ * gathered per 64 x 4 object tile in position order (no atomics, deterministic).
  * Without chi_slices it runs after phx_lsqml_backward(i) and before the step of slice
  * i-1 (chi ping-pongs).  With chi_slices it may run after the whole backward sweep: the
  * sweep reads only the pre-update cutouts (solvers.py:246-260), so deferring every slice's

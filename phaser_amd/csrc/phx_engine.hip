@@ -481,6 +481,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const RedLayout L = red_layout(S, bmax, M);
 
+  float2 x[E];
   // far field: chi_hat = (sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask F
   const float2* F = (const float2*)d.farfield + ((size_t)b * M + m) * NN;
   const float* I = d.intensity + (size_t)b * NN;
@@ -488,7 +489,6 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   const float* mask = d.mask_s;
   const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
   float e2 = 0.f;
-  float2 x[E];
   pl.cols_apply(x, [&](int py, int px, float2) {
     const int e = py * N + px;
     const float Iv = I[e], Dv = D[e], mk = mask[e];
@@ -578,13 +578,14 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
 // LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
 //   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
 //   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
-// One workgroup owns a 16x16 object tile of slice first + blockIdx.z (one pixel per
-// thread).  Wave 0 lists the group's positions whose cutout meets the tile, in group
+// One workgroup owns a 64 x 4 object tile of slice first + blockIdx.z (one pixel per
+// thread; a wave reads 512 contiguous bytes of a plane row, so the arbitrary offset of a
+// cutout inside its 128-byte lines costs 1.25x, not the 2x of 16-pixel rows).  Wave 0 lists the group's positions whose cutout meets the tile, in group
 // order (ballot + prefix count); then every thread gathers both sums from those
 // positions' chi_i and psi_i planes, positions in group order and modes in order: no
 // atomics, and the result does not depend on scheduling.  Pixels no cutout covers are
 // left untouched (their reference update is exactly zero).
-constexpr int UPD_TILE = 16;
+constexpr int UPD_TW = 64, UPD_TH = 4;
 constexpr int UPD_LIST = 256;  // positions listed per pass
 __global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int first) {
   const int i = first + (int)blockIdx.z;
@@ -605,8 +606,8 @@ __global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, i
   }
   const float reg = d.params[P_REG_OBJ];
   const int n = d.n, NN = n * n, M = d.n_modes, H = d.obj_h, W = d.obj_w;
-  const int ty0 = blockIdx.y * UPD_TILE, tx0 = blockIdx.x * UPD_TILE;
-  const int y = ty0 + threadIdx.x / UPD_TILE, x = tx0 + threadIdx.x % UPD_TILE;
+  const int ty0 = blockIdx.y * UPD_TH, tx0 = blockIdx.x * UPD_TW;
+  const int y = ty0 + threadIdx.x / UPD_TW, x = tx0 + threadIdx.x % UPD_TW;
   const float2* chi = chi_in_buf(d, i);
   const float2* psi = (const float2*)d.psi;
   float2 dO = make_float2(0.f, 0.f);
@@ -624,7 +625,7 @@ __global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, i
           const int p = d.group_pos[b];
           y0 = d.start[2 * p];
           x0 = d.start[2 * p + 1];
-          ov = !(y0 >= ty0 + UPD_TILE || y0 + n <= ty0 || x0 >= tx0 + UPD_TILE || x0 + n <= tx0);
+          ov = !(y0 >= ty0 + UPD_TH || y0 + n <= ty0 || x0 >= tx0 + UPD_TW || x0 + n <= tx0);
         }
         const uint64_t msk = __ballot(ov);
         if (ov) {
@@ -639,6 +640,7 @@ __global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, i
     }
     __syncthreads();
     const int cnt = s_cnt;
+#pragma unroll 2
     for (int t = 0; t < cnt; ++t) {
       const int b = s_b[t];
       const int ly = y - s_y0[t], lx = x - s_x0[t];
@@ -1264,7 +1266,7 @@ int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count
   if (first < 0 || count < 1 || first + count > d->n_slices || !d->chi_a || !d->chi_b || !d->finite_flag ||
       (count > 1 && !d->chi_slices))
     return PHX_EINVAL;
-  const dim3 grid((unsigned)((d->obj_w + UPD_TILE - 1) / UPD_TILE), (unsigned)((d->obj_h + UPD_TILE - 1) / UPD_TILE),
+  const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH),
                   (unsigned)count);
   hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, first);
   PHX_CHECK_LAUNCH();

@@ -265,8 +265,10 @@ class GroupWorkspace:
                 c("phx_probe_spectrum_reduce", d, sh)
                 c("phx_lsqml_probe_apply", d, sh)
             if update_object:
+                # after the position step, which reads the pre-update slice-0 cutout
+                # (solvers.py:275-288)
                 c("phx_lsqml_object_update_slices", d, 0, self.S, st)
-            if update_probe and side is not None:
+            if side is not None:
                 main.wait_stream(side)
             return
         # illumination sums are gathered by the object update, not scattered here

@@ -144,15 +144,15 @@ def test_lsqml_position_correction():
     assert e_g64 <= 3 * e_o64 + 1e-5
 
 
-@pytest.mark.parametrize('solver', ['lsqml', 'epie'])
-def test_fused_sweeps_and_deferred_updates_equal_per_slice_launches(solver):
+@pytest.mark.parametrize('solver,slices', [('lsqml', 3), ('epie', 3), ('lsqml', 5)])
+def test_fused_sweeps_and_deferred_updates_equal_per_slice_launches(solver, slices):
     """Whole-sweep kernels (one launch for the forward model, one for the LSQ-ML backward
     sweep, every object update after the sweep) give the per-slice launch sequence's
     result: the sweep reads only pre-update cutouts (solvers.py:246-260)."""
     from phaser_amd.engines.conventional import run_engine
     from phaser_amd.plan import ConventionalEnginePlan, MomentumPositionSolverPlan
-    _, state, pats = _problem(n=32, n_modes=2, n_slices=3, scan_shape=(6, 6), scan_step_px=3.0, jitter_px=0.5,
-                              seed=5)
+    _, state, pats = _problem(n=32, n_modes=2, n_slices=slices, scan_shape=(6, 6), scan_step_px=3.0,
+                              jitter_px=0.5, seed=5)
     plan = ConventionalEnginePlan(noise_model='amplitude', solver=solver, niter=2, grouping=12,
                                   position_solver=MomentumPositionSolverPlan(step_size=1e-2),
                                   update_positions=True)
@@ -168,4 +168,7 @@ def test_fused_sweeps_and_deferred_updates_equal_per_slice_launches(solver):
                       rel_err(o.scan, ref.scan))
         print(f"{kw}: object {eo:.2e} probe {ep:.2e} scan {es:.2e}", flush=True)
         assert eo < 1e-6 and ep < 1e-6 and es < 1e-6
-        numpy.testing.assert_allclose(o.progress.detector_errors, ref.progress.detector_errors, rtol=1e-6)
+        # ePIE scatters its object update with float atomics (order-dependent rounding),
+        # which moves the next iteration's error at the 1e-6 level
+        numpy.testing.assert_allclose(o.progress.detector_errors, ref.progress.detector_errors,
+                                      rtol=1e-5 if solver == 'epie' else 1e-6)
