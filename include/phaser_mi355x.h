@@ -227,6 +227,17 @@ int phx_farfield_gradient(const phx_group_desc* d, int poisson, hipStream_t stre
  * the probe-gradient spectrum in d_probe and the position gradients in red.pos_num. */
 int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, int want_probe, int want_positions,
                           hipStream_t stream);
+/* The far-field gradient update alone (after phx_group_forward): chi_a = dL/d(exit wave)
+ * and the group loss, as the second half of phx_farfield_gradient. */
+int phx_farfield_gradient_update(const phx_group_desc* d, int poisson, hipStream_t stream);
+/* Fused adjoint sweep (gradient/run.py:293-349 through the hand-written adjoint of the
+ * forward model): from the exit-wave gradient in chi_a, every slice in one launch with
+ * the gradient carried in registers: grad_obj[i] += scatter(g conj(psi_i)), g <- g
+ * conj(O_i), g <- ifft2(fft2(g) conj(prop_{i-1})); at slice 0 the probe-gradient
+ * spectrum (d_probe) and the position gradients (red pos_num).  Needs every psi_i
+ * stored (not the recompute mode). */
+int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe, int want_positions,
+                       hipStream_t stream);
 /* grad_probe = ifft2(d_probe) (ortho); d_probe = 0. */
 int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream);
 /* Gradient-engine rescale factors into errors[b]: sum(D) / sum |psi_{S-1} O_{S-1}|^2

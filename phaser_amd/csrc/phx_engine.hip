@@ -575,6 +575,91 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   }
 }
 
+// Fused adjoint sweep of the gradient engine (the reverse of group_forward_k; cf. the
+// per-slice gradient_backward_k): starting from the exit-wave gradient in chi_a (written
+// by the far-field gradient update), for i = S-1 .. 0 with the gradient carried in
+// registers: grad_O_i += scatter(g conj(psi_i)) (atomics), g <- g conj(O_i), and
+// g <- ifft2(fft2(g) conj(p_{i-1})); at slice 0 the probe-gradient spectrum and the
+// position gradients.  Needs every psi_i stored.
+template <int N, int T, bool WANT_OBJ, bool WANT_P0>
+__global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want_probe, int want_pos) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr int E = Plane<N>::E;
+  constexpr int G = E < 8 ? E : 8;  // points per load group
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int M = d.n_modes, S = d.n_slices;
+  const int p = d.group_pos[b];
+  const Cut c = cut_of(d, p);
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  float2 x[E];
+  {
+    // the stored exit-wave gradient, times n^2 (a power of two: exact) to carry it unscaled
+    const float2* g0 = (const float2*)d.chi_a + ((size_t)b * M + m) * NN;
+    pl.rows_apply(x, [&](int yy, int xx, float2) { return cscale(g0[yy * N + xx], (float)NN); }, [] {});
+  }
+  for (int i = S - 1; i >= 0; --i) {
+    pl.refresh();
+    const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
+    const float2* obj_i = (const float2*)d.object + i * HW;
+    float2* gO = (float2*)d.grad_obj + i * HW;
+    const int yy = pl.rc;
+#pragma unroll
+    for (int j0 = 0; j0 < E; j0 += G) {
+      // all of a group's loads before its atomics (which the compiler will not reorder)
+      float2 ps[G], ob[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int xx = 4 * (j0 + u) + pl.q;
+        if (WANT_OBJ) ps[u] = psi_i[yy * N + xx];
+        ob[u] = obj_at(obj_i, c, yy, xx);
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int xx = 4 * (j0 + u) + pl.q;
+        const float2 gv = cscale(x[j0 + u], inv_nn);
+        if (WANT_OBJ && c.inb(yy, xx)) atomic_add_c64(&gO[c.idx(yy, xx)], cmulc(gv, ps[u]));
+        x[j0 + u] = cmulc(gv, ob[u]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (i == 0) break;
+    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); });
+  }
+  if (!WANT_P0) return;
+  // slice 0: probe-gradient spectrum and position gradients (as gradient_backward_k)
+  const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+  const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
+  float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+  float gy = 0.f, gx = 0.f;  // per-lane FP32 partials, FP64 block sums
+  pl.forward_regs(x);
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int py = 4 * k + pl.q, px = pl.rc;
+    const int e = py * N + px;
+    const float2 f = shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]);
+    if (want_probe) atomic_add_c64(&dPk[e], cmulc(x[k], f));
+    if (want_pos) {
+      const float2 t = cmulc(cmul(Pk[e], f), x[k]);  // conj(v) * fft2(P) filt
+      gy += TWO_PI * d.ky_s[py] * t.y;
+      gx += TWO_PI * d.kx_s[px] * t.y;
+    }
+    if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+  }
+  if (want_pos) {
+    const double ty = block_sum_f64(gy, red_lds_);
+    const double tx = block_sum_f64(gx, red_lds_);
+    if (threadIdx.x == 0) {
+      const RedLayout L = red_layout(S, d.bmax, M);
+      atomic_add_f64(&d.red[L.pos_num + 2 * b], ty * (double)inv_nn);
+      atomic_add_f64(&d.red[L.pos_num + 2 * b + 1], tx * (double)inv_nn);
+    }
+  }
+}
+
 // LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
 //   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
 //   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
@@ -1252,6 +1337,53 @@ int phx_lsqml_backward_sweep(const phx_group_desc* d, int update_object, int upd
       K(N, T, false, true);           \
     else                              \
       K(N, T, false, false);          \
+  } while (0)
+  PHX_DISPATCH(d->n, L);
+#undef L
+#undef K
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_farfield_gradient_update(const phx_group_desc* d, int poisson, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a) return PHX_EINVAL;
+#define L(N, T)                                                                                                  \
+  do {                                                                                                           \
+    if (poisson)                                                                                                 \
+      hipLaunchKernelGGL((farfield_update_k<N, T, FM_GRAD_POISSON>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0);   \
+    else                                                                                                         \
+      hipLaunchKernelGGL((farfield_update_k<N, T, FM_GRAD_AMPLITUDE>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, 0); \
+  } while (0)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe, int want_positions,
+                       hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->chi_a || (d->n_slices > 1 && !d->props) || (want_object && !d->grad_obj) ||
+      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)))
+    return PHX_EINVAL;
+  if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // needs every psi_i
+  const bool p0 = want_probe || want_positions;
+#define K(N, T, WO, WP) \
+  hipLaunchKernelGGL((gradient_sweep_k<N, T, WO, WP>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, want_probe, \
+                     want_positions)
+#define L(N, T)                 \
+  do {                          \
+    if (want_object && p0)      \
+      K(N, T, true, true);      \
+    else if (want_object)       \
+      K(N, T, true, false);     \
+    else if (p0)                \
+      K(N, T, false, true);     \
+    else                        \
+      K(N, T, false, false);    \
   } while (0)
   PHX_DISPATCH(d->n, L);
 #undef L

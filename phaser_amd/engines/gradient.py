@@ -50,11 +50,13 @@ def _p(t):
 class MultisliceModel:
     """Per-group loss and gradient of the multislice model on a GroupWorkspace."""
 
-    def __init__(self, ws: GroupWorkspace, poisson: bool):
+    def __init__(self, ws: GroupWorkspace, poisson: bool, fused=None):
         self.ws = ws
         self.poisson = int(poisson)
         self.L = red_layout(ws.S, ws.B, ws.M)
         self._pending = None
+        # whole-sweep kernels need every slice's wave stored (not the recompute mode)
+        self.fused = (ws.psi_slots >= ws.S) if fused is None else (bool(fused) and ws.psi_slots >= ws.S)
 
     def dry_run(self, g):
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
@@ -71,10 +73,15 @@ class MultisliceModel:
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         _lib.call("phx_group_begin", d, st)
         _lib.call("phx_probe_spectra", d, st)
-        _lib.call("phx_shift_probes", d, 0, int(ws.S > 1), st)
-        for i in range(1, ws.S - 1):
-            _lib.call("phx_forward_slice", d, i, 0, st)
-        _lib.call("phx_farfield_gradient", d, self.poisson, st)
+        if self.fused:
+            # every slice in one launch, then the far-field gradient and loss
+            _lib.call("phx_group_forward", d, st)
+            _lib.call("phx_farfield_gradient_update", d, self.poisson, st)
+        else:
+            _lib.call("phx_shift_probes", d, 0, int(ws.S > 1), st)
+            for i in range(1, ws.S - 1):
+                _lib.call("phx_forward_slice", d, i, 0, st)
+            _lib.call("phx_farfield_gradient", d, self.poisson, st)
         self._pending = g
 
     def loss_tensor(self, g):
@@ -88,6 +95,12 @@ class MultisliceModel:
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         if want_object:
             _lib.call("phx_fill_u32", _p(ws.grad_obj), 0, ws.grad_obj.numel() * 2, st)
+        if self.fused:
+            _lib.call("phx_gradient_sweep", d, int(want_object), int(want_probe), int(want_positions), st)
+            if want_probe:
+                _lib.call("phx_gradient_probe_finish", d, st)
+            self._pending = None
+            return
         recompute = ws.psi_slots < ws.S
         for i in range(ws.S - 1, -1, -1):
             if recompute and 1 <= i <= ws.S - 3:
