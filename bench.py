@@ -212,6 +212,14 @@ def main():
     alg = kernel_bytes(dom, B, M, S, NN, HW)
     achieved = alg / (kt[dom]['avg_us'] * 1e-6) / 1e9
     per_pi = lsqml_bytes_per_pattern_iter(S, M, NN, HW, B)
+    # measured HBM bytes per launch of that kernel (two rocprofv3 counter runs, scaled by
+    # a copy of known size; committed with the command that made them)
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "lsqml_multislice_512_b64_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)['kernels'].get(dom, {}).get('traffic_bytes')
     whole = per_pi * value / 1e9
     log(f"kernels: {json.dumps(kt)}")
 
@@ -248,7 +256,7 @@ def main():
                                "x 4 modes, 4096 patterns 128x128 f32 (Poisson 1e5), groups of 64, 1 step = 1 "
                                "iteration over all patterns", "graphs": not args.no_graphs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg, "avg_launch_us": kt[dom]['avg_us'],
                      "whole_iteration_alg_GBs": whole, "whole_iteration_frac": whole / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
