@@ -73,7 +73,7 @@ __global__ void fill_u32_k(uint32_t* p, uint32_t v, int64_t n) {
 template <int N, int T>
 __global__ void __launch_bounds__(T) fft2_conv_k(const float2* __restrict__ in, float2* __restrict__ out,
                                                  const float2* __restrict__ twg, int inverse) {
-  static_assert(T == Plane<N>::T, "plane kernels run 4n threads");
+  static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");
   __shared__ float2 lds[Plane<N>::LDS_FLOAT2 + N];
   Plane<N> pl(lds, lds + Plane<N>::LDS_FLOAT2);
   load_twiddles<N>(lds + Plane<N>::LDS_FLOAT2, twg);
@@ -203,7 +203,7 @@ int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int invers
     case 16: return launch_fft2<16, 64>(in, out, batch, inverse, twiddle, stream);
     case 32: return launch_fft2<32, 128>(in, out, batch, inverse, twiddle, stream);
     case 64: return launch_fft2<64, 256>(in, out, batch, inverse, twiddle, stream);
-    case 128: return launch_fft2<128, 512>(in, out, batch, inverse, twiddle, stream);
+    case 128: return launch_fft2<128, 1024>(in, out, batch, inverse, twiddle, stream);
     default: return PHX_EUNSUPPORTED;
   }
 }

@@ -77,7 +77,7 @@ __device__ __forceinline__ float2 shift_filter(float sy, float sx, float ky, flo
 }
 
 #define PHX_PLANE_SETUP(N, T)                                                    \
-  static_assert(T == Plane<N>::T, "plane kernels run 4n threads");               \
+  static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");               \
   __shared__ float2 lds_[Plane<N>::LDS_FLOAT2 + N];                              \
   __shared__ double red_lds_[T / 64];                                            \
   Plane<N> pl(lds_, lds_ + Plane<N>::LDS_FLOAT2);                                \
@@ -145,14 +145,14 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int propag
   float2 x[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
-    const int py = 4 * k + pl.q, px = pl.rc;
+    const int py = pl.QL * k + pl.q, px = pl.rc;
     x[k] = cmul(Pk[py * N + px], shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]));
   }
   pl.inverse_regs(x);
   // x now holds psi0 (unscaled) at row rc, columns 4j + q
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    const int yy = pl.rc, xx = 4 * j + pl.q;
+    const int yy = pl.rc, xx = pl.QL * j + pl.q;
     const float2 v = cscale(x[j], inv_nn);
     psi0[yy * N + xx] = v;
     const float a = abs2f(v);
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
   const bool single = M == 1;
 #pragma unroll
   for (int k = 0; k < E; ++k) {
-    const int e = (4 * k + pl.q) * N + pl.rc;
+    const int e = (pl.QL * k + pl.q) * N + pl.rc;
     F[e] = x[k];
     const float a = abs2f(x[k]) * inv_nn;
     if (single) I[e] = a;
@@ -568,7 +568,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
     const float kx = d.kx_s[px];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int py = 4 * k + pl.q;
+      const int py = pl.QL * k + pl.q;
       Fp[py * N + px] = cmulc(x[k], shift_filter(sy, sx, d.ky_s[py], kx));
       if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // a few phase factors at a time
     }
@@ -612,13 +612,13 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
       float2 ps[G], ob[G];
 #pragma unroll
       for (int u = 0; u < G; ++u) {
-        const int xx = 4 * (j0 + u) + pl.q;
+        const int xx = pl.QL * (j0 + u) + pl.q;
         if (WANT_OBJ) ps[u] = psi_i[yy * N + xx];
         ob[u] = obj_at(obj_i, c, yy, xx);
       }
 #pragma unroll
       for (int u = 0; u < G; ++u) {
-        const int xx = 4 * (j0 + u) + pl.q;
+        const int xx = pl.QL * (j0 + u) + pl.q;
         const float2 gv = cscale(x[j0 + u], inv_nn);
         if (WANT_OBJ && c.inb(yy, xx)) atomic_add_c64(&gO[c.idx(yy, xx)], cmulc(gv, ps[u]));
         x[j0 + u] = cmulc(gv, ob[u]);
@@ -638,7 +638,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   pl.forward_regs(x);
 #pragma unroll
   for (int k = 0; k < E; ++k) {
-    const int py = 4 * k + pl.q, px = pl.rc;
+    const int py = pl.QL * k + pl.q, px = pl.rc;
     const int e = py * N + px;
     const float2 f = shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]);
     if (want_probe) atomic_add_c64(&dPk[e], cmulc(x[k], f));
@@ -1150,7 +1150,7 @@ int check_desc(const phx_group_desc* d) {
     case 16: LAUNCH(16, 64); break;   \
     case 32: LAUNCH(32, 128); break;  \
     case 64: LAUNCH(64, 256); break;  \
-    case 128: LAUNCH(128, 512); break;\
+    case 128: LAUNCH(128, 1024); break;\
     default: return PHX_EUNSUPPORTED; \
   }
 

@@ -1,21 +1,26 @@
 // Batched 2-D complex FFT of n x n planes (n = 16, 32, 64, 128) for gfx950, one plane
-// per workgroup of 4n threads (n/16 wavefronts).
+// per workgroup of L n threads.
 //
-// Layout.  Four lanes (a "quad") own one row (row phase) or one column (column phase) of
-// the plane; lane q of the quad holds the E = n/4 points with index 4 j + q, j < E.
-// A length-n transform along the quad is a four-step n = E * 4 decomposition:
+// Layout.  A group of L lanes (L = 4 for n <= 64, 8 for n = 128) owns one row (row
+// phase) or one column (column phase) of the plane; lane q holds the E = n/L points
+// with index L j + q, j < E.  A length-n transform along the group is a four-step
+// n = E * L decomposition:
 //   1. in-register DFT_E over j (radix-2, compile-time twiddles),
 //   2. twiddle W_n^(q k1) (table in LDS),
-//   3. DFT_4 across the quad in two radix-2 stages of DPP quad permutes (no LDS).
-// Lane q then holds X[k1 + E bitrev2(q)], k1 < E, which it stores at "scrambled"
-// position p = 4 k1 + q: position p holds frequency f(p) = p/4 + E bitrev2(p % 4).
-// The inverse runs the same steps backwards and returns natural order.
+//   3. DFT_L across the lanes: for L = 8 a radix-2 exchange with lane q^4 plus the
+//      W_8 twiddle, then (for both) the DFT_4 of each quad as two radix-2 exchanges.
+//      All exchanges are DPP permutes (no LDS).
+// Lane q then holds X[k1 + E bitrev(q)], k1 < E, which it stores at "scrambled"
+// position p = L k1 + q: position p holds frequency f(p) = p/L + E bitrev(p % L).
+// The inverse runs the same steps backwards and returns natural order.  At n = 128 the
+// 8-lane layout gives 1024 threads per plane: with one plane per CU (its LDS), that is
+// 4 waves per SIMD to hide each phase's latency instead of 2.
 //
 // A 2-D transform is: rows in registers -> transpose through LDS -> columns in registers.
-// The LDS plane has row stride n + 4 float2, which makes both the row-phase writes
-// (s[r][4k + q]) and the column-phase reads (s[4j + q][c]) bank-conflict free.  A forward
-// transform, a pointwise multiply in the (scrambled) frequency domain and an inverse
-// transform therefore cost two LDS round trips in total.
+// The LDS plane has row stride n + 4 float2, which makes the column-phase accesses
+// (s[L j + q][c]) bank-conflict free, and the row-phase ones (s[r][L k + q]) as well for
+// L = 4 (2-way for L = 8).  A forward transform, a pointwise multiply in the (scrambled)
+// frequency domain and an inverse transform therefore cost two LDS round trips in total.
 //
 // Transforms are unnormalised: forward uses exp(-2 pi i nk/n), inverse exp(+2 pi i nk/n),
 // so inverse(forward(a)) = n^2 a; callers scale by 1/n^2.  The (i)fftshift of the centred
@@ -102,31 +107,48 @@ __device__ __forceinline__ float dpp_xor1(float v) {
 __device__ __forceinline__ float dpp_xor2(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
 }
+// lane q ^ 4 within each group of 8: row_half_mirror (lane p <- 7 - p within 8) then
+// quad_perm [3,2,1,0] (p <- p ^ 3 within 4): 7 - (p ^ 3) = p ^ 4
+__device__ __forceinline__ float dpp_xor4(float v) {
+  const int m = __builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false);
+  return __int_as_float(__builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, false));
+}
 __device__ __forceinline__ float2 quad_xor1(float2 v) { return make_float2(dpp_xor1(v.x), dpp_xor1(v.y)); }
 __device__ __forceinline__ float2 quad_xor2(float2 v) { return make_float2(dpp_xor2(v.x), dpp_xor2(v.y)); }
+__device__ __forceinline__ float2 oct_xor4(float2 v) { return make_float2(dpp_xor4(v.x), dpp_xor4(v.y)); }
 
-// Frequency index held at scrambled position p of an N-point axis.
+// Lanes per row (row phase) or column (column phase): 8 at n = 128 (1024 threads: 4
+// waves per SIMD for the one plane a CU holds), 4 below.
+template <int N>
+struct PlaneLanes {
+  static constexpr int value = N >= 128 ? 8 : 4;
+};
+
+// Frequency index held at scrambled position p of an N-point axis: with L lanes, lane q
+// ends holding X[k + (N/L) bitrev(q)] at position L k + q (bitrev over log2 L bits).
 template <int N>
 __host__ __device__ __forceinline__ int scr_to_freq(int p) {
-  const int q = p & 3;
-  return (p >> 2) + (N / 4) * (((q & 1) << 1) | (q >> 1));
+  constexpr int L = PlaneLanes<N>::value;
+  const int q = p % L;
+  const int br = L == 4 ? (((q & 1) << 1) | (q >> 1)) : (((q & 1) << 2) | (q & 2) | (q >> 2));
+  return p / L + (N / L) * br;
 }
 
 template <int N>
 struct Plane {
-  static constexpr int E = N / 4;       // points per lane
-  static constexpr int T = 4 * N;       // threads per workgroup
+  static constexpr int QL = PlaneLanes<N>::value;  // lanes per row / column
+  static constexpr int E = N / QL;      // points per lane
+  static constexpr int T = QL * N;      // threads per workgroup
   static constexpr int LD = N + 4;      // LDS row stride (float2)
   static constexpr int LDS_FLOAT2 = N * LD;
   float2* s;         // N x LD
   const float2* tw;  // W_N^e, e < N, in LDS
-  int rc;            // row (row phase) or column (column phase) owned by this quad
-  int q;             // lane within the quad
+  int rc;            // row (row phase) or column (column phase) owned by this lane group
+  int q;             // lane within the group
 
   __device__ __forceinline__ Plane(float2* s_, const float2* tw_) : s(s_), tw(tw_) {
-    const int l = threadIdx.x & 63;
-    rc = 16 * (threadIdx.x >> 6) + (l >> 2);
-    q = l & 3;
+    rc = threadIdx.x / QL;
+    q = threadIdx.x % QL;
   }
 
   // Makes rc and q opaque for one loop iteration: called at the top of a loop over
@@ -134,40 +156,73 @@ struct Plane {
   // the loop (dozens of registers live across all transforms).
   __device__ __forceinline__ void refresh() { __asm__ volatile("" : "+v"(rc), "+v"(q)); }
 
-  // ----- one 1-D transform along the quad (registers + DPP only) -----
-  // The DFT-4 across the quad is two radix-2 exchanges (partner q^2, then q^1), each
-  // written as one FMA per component with a per-lane sign (+-1 is exact), and the one
-  // rotation of the radix-2x2 split (by -i on lane 3 in the forward, +i in the inverse,
-  // applied before the exchange that reads it) as a select.  No lane-dependent branches:
-  // a quad's lanes run the same instructions.
+  // ----- one 1-D transform along the lane group (registers + DPP only) -----
+  // After the in-lane DFT and the twiddle W_N^(q k), a DFT across the lanes:
+  //  * with 8 lanes, first a radix-2 exchange with lane q^4 (y = +-x + partner) and the
+  //    twiddle W_8^(q-4) on the upper four lanes;
+  //  * then, on each quad, the DFT-4 as two radix-2 exchanges (partner q^2, then q^1),
+  //    each one FMA per component with a per-lane sign (+-1 is exact), and the one
+  //    rotation of the radix-2x2 split (by -i on lane 3 of the quad in the forward, +i in
+  //    the inverse, applied before the exchange that reads it) as a select.
+  // No lane-dependent branches: a group's lanes run the same instructions.
+  __device__ __forceinline__ float2 w8_lane() const {
+    // W_8^(q-4) for the upper lanes of an 8-lane group, 1 for the lower ones
+    constexpr float c = 0.70710678118654752f;
+    const int r = q & 3;
+    const bool up = q >= 4;
+    const float wr = !up ? 1.f : (r == 0 ? 1.f : (r == 1 ? c : (r == 2 ? 0.f : -c)));
+    const float wi = !up ? 0.f : (r == 0 ? 0.f : (r == 2 ? -1.f : -c));
+    return make_float2(wr, wi);
+  }
   __device__ __forceinline__ void fwd(float2 (&x)[E]) {
     dft<E, false>(x);
     // tw[0] = 1 exactly, so lane 0 multiplies by 1
 #pragma unroll
     for (int k = 1; k < E; ++k) x[k] = cmul(x[k], tw[q * k]);
-    const float s1 = q < 2 ? 1.f : -1.f;   // y = x + p (q < 2), p - x (q >= 2)
-    const float s2 = (q & 1) ? -1.f : 1.f; // out = y + p (even), p - y (odd)
-    const bool r3 = q == 3;
+    if (QL == 8) {
+      const float s4 = q < 4 ? 1.f : -1.f;
+      const float2 w = w8_lane();
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const float2 p = oct_xor4(x[k]);
+        x[k] = cmul(make_float2(fmaf(s4, x[k].x, p.x), fmaf(s4, x[k].y, p.y)), w);
+      }
+    }
+    const int qq = q & 3;
+    const float s1 = qq < 2 ? 1.f : -1.f;   // y = x + p (qq < 2), p - x (qq >= 2)
+    const float s2 = (qq & 1) ? -1.f : 1.f; // out = y + p (even), p - y (odd)
+    const bool r3 = qq == 3;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const float2 p = quad_xor2(x[k]);
       const float yr = fmaf(s1, x[k].x, p.x), yi = fmaf(s1, x[k].y, p.y);
-      const float2 y = make_float2(r3 ? yi : yr, r3 ? -yr : yi);  // lane 3: * W4^1 = -i
+      const float2 y = make_float2(r3 ? yi : yr, r3 ? -yr : yi);  // * W4^1 = -i
       const float2 p2 = quad_xor1(y);
       x[k] = make_float2(fmaf(s2, y.x, p2.x), fmaf(s2, y.y, p2.y));
     }
   }
   __device__ __forceinline__ void inv(float2 (&x)[E]) {
-    const float s1 = q < 2 ? 1.f : -1.f;
-    const float s2 = (q & 1) ? -1.f : 1.f;
-    const bool r3 = q == 3;
+    const int qq = q & 3;
+    const float s1 = qq < 2 ? 1.f : -1.f;
+    const float s2 = (qq & 1) ? -1.f : 1.f;
+    const bool r3 = qq == 3;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const float2 p = quad_xor1(x[k]);
       const float ar = fmaf(s2, x[k].x, p.x), ai = fmaf(s2, x[k].y, p.y);
-      const float2 a = make_float2(r3 ? -ai : ar, r3 ? ar : ai);  // lane 3: * conj(W4^1) = +i
+      const float2 a = make_float2(r3 ? -ai : ar, r3 ? ar : ai);  // * conj(W4^1) = +i
       const float2 p2 = quad_xor2(a);
       x[k] = make_float2(fmaf(s1, a.x, p2.x), fmaf(s1, a.y, p2.y));
+    }
+    if (QL == 8) {
+      const float s4 = q < 4 ? 1.f : -1.f;
+      const float2 w = w8_lane();
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const float2 a = cmulc(x[k], w);
+        const float2 p = oct_xor4(a);
+        x[k] = make_float2(fmaf(s4, a.x, p.x), fmaf(s4, a.y, p.y));
+      }
     }
 #pragma unroll
     for (int k = 1; k < E; ++k) x[k] = cmulc(x[k], tw[q * k]);
@@ -175,24 +230,25 @@ struct Plane {
   }
 
   // ----- transposes through LDS -----
-  // row phase (x[k] at scrambled column 4k+q of row rc) -> column phase (x[j] at row 4j+q
-  // of column rc)
+  // row phase (x[k] at scrambled column QL k + q of row rc) -> column phase (x[j] at row
+  // QL j + q of column rc)
   __device__ __forceinline__ void rows_to_cols(float2 (&x)[E]) {
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < E; ++k) s[rc * LD + 4 * k + q] = x[k];
+    for (int k = 0; k < E; ++k) s[rc * LD + QL * k + q] = x[k];
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < E; ++j) x[j] = s[(4 * j + q) * LD + rc];
+    for (int j = 0; j < E; ++j) x[j] = s[(QL * j + q) * LD + rc];
   }
-  // column phase (x[j] at row 4j+q of column rc) -> row phase (x[k] at column 4k+q of row rc)
+  // column phase (x[j] at row QL j + q of column rc) -> row phase (x[k] at column QL k + q
+  // of row rc)
   __device__ __forceinline__ void cols_to_rows(float2 (&x)[E]) {
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < E; ++j) s[(4 * j + q) * LD + rc] = x[j];
+    for (int j = 0; j < E; ++j) s[(QL * j + q) * LD + rc] = x[j];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < E; ++k) x[k] = s[rc * LD + 4 * k + q];
+    for (int k = 0; k < E; ++k) x[k] = s[rc * LD + QL * k + q];
   }
 
   // ----- composites -----
@@ -204,7 +260,7 @@ struct Plane {
   __device__ __forceinline__ void load_rows(Src src, float2 (&x)[E], Pin pin) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      x[j] = src(rc, 4 * j + q);
+      x[j] = src(rc, QL * j + q);
       if ((j % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || j == E - 1) {
         pin();
         // the group's global loads first, then its arithmetic
@@ -223,7 +279,7 @@ struct Plane {
   __device__ __forceinline__ void rows_apply(float2 (&x)[E], F f, Pin pin) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      x[j] = f(rc, 4 * j + q, x[j]);
+      x[j] = f(rc, QL * j + q, x[j]);
       if ((j % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || j == E - 1) {
         pin();
         __builtin_amdgcn_sched_group_barrier(0x0020, 4 * PHX_LOAD_PACE, 0);
@@ -237,7 +293,7 @@ struct Plane {
   __device__ __forceinline__ void cols_apply(float2 (&x)[E], F f, Pin pin) {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      x[k] = f(4 * k + q, rc, x[k]);
+      x[k] = f(QL * k + q, rc, x[k]);
       if ((k % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || k == E - 1) {
         pin();
         __builtin_amdgcn_sched_group_barrier(0x0020, 4 * PHX_LOAD_PACE, 0);
@@ -250,9 +306,9 @@ struct Plane {
   template <class Dst>
   __device__ __forceinline__ void store_rows(Dst dst, float2 (&x)[E]) {
 #pragma unroll
-    for (int j = 0; j < E; ++j) dst(rc, 4 * j + q, x[j]);
+    for (int j = 0; j < E; ++j) dst(rc, QL * j + q, x[j]);
   }
-  // row-phase natural data -> column-phase scrambled spectrum (py = 4k+q, px = rc)
+  // row-phase natural data -> column-phase scrambled spectrum (py = QL k + q, px = rc)
   __device__ __forceinline__ void forward_regs(float2 (&x)[E]) {
     fwd(x);
     rows_to_cols(x);
@@ -271,7 +327,7 @@ struct Plane {
     load_rows(src, x, pin);
     forward_regs(x);
 #pragma unroll
-    for (int k = 0; k < E; ++k) fn(4 * k + q, rc, x[k]);
+    for (int k = 0; k < E; ++k) fn(QL * k + q, rc, x[k]);
   }
   template <class Src, class Fn>
   __device__ __forceinline__ void forward(Src src, Fn fn) {
@@ -283,7 +339,7 @@ struct Plane {
     float2 x[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      x[k] = src(4 * k + q, rc);
+      x[k] = src(QL * k + q, rc);
       if ((k % PHX_LOAD_PACE) == PHX_LOAD_PACE - 1 || k == E - 1) {
         pin();
         __builtin_amdgcn_sched_group_barrier(0x0020, 4 * PHX_LOAD_PACE, 0);
@@ -304,7 +360,7 @@ struct Plane {
     forward_regs(x);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) x[k] = mul(4 * k + q, rc, x[k]);
+    for (int k = 0; k < E; ++k) x[k] = mul(QL * k + q, rc, x[k]);
     // the multiplier's loads (a propagator: L2-resident) all issued before the products
     __builtin_amdgcn_sched_group_barrier(0x0020, 2 * E, 0);
     __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);

@@ -71,16 +71,26 @@ def make_propagators(sampling, wavelength, thicknesses, bwlim_frac=2 / 3, dtype=
     return numpy.stack(out, axis=0)
 
 
+def fft_lanes(n: int) -> int:
+    """Lanes per row/column of the FFT kernels for an n-point axis (8 at n = 128)."""
+    return 8 if n >= 128 else 4
+
+
 def scr_to_freq(n: int) -> numpy.ndarray:
     """Frequency index held at each scrambled position of an n-point axis.
 
-    Lane q of a quad ends the transform holding X[k + (n/4) bitrev2(q)] at position
-    4k + q, so position p holds frequency p // 4 + (n / 4) * bitrev2(p % 4)."""
+    With L lanes per row, lane q ends the transform holding X[k + (n/L) bitrev(q)] at
+    position L k + q (bitrev over log2(L) bits)."""
     if n not in FFT_SIZES:
         raise ValueError(f"unsupported plane side {n}; the FFT kernels support {FFT_SIZES}")
+    L = fft_lanes(n)
     p = numpy.arange(n)
-    q = p % 4
-    return p // 4 + (n // 4) * (((q & 1) << 1) | (q >> 1))
+    q = p % L
+    if L == 4:
+        br = ((q & 1) << 1) | (q >> 1)
+    else:
+        br = ((q & 1) << 2) | (q & 2) | (q >> 2)
+    return p // L + (n // L) * br
 
 
 def scramble(a: numpy.ndarray) -> numpy.ndarray:

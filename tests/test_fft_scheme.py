@@ -1,15 +1,17 @@
-"""CPU model of the index scheme of the quad FFT kernels (phaser_amd/csrc/phx_fft.h).
+"""CPU model of the index scheme of the FFT kernels (phaser_amd/csrc/phx_fft.h).
 
-Four lanes (a quad) own one row or column; lane q holds points 4j+q.  Per lane: an
-E = n/4 point DFT over j, a twiddle W_n^(q k), then a DFT_4 across the quad in two
-radix-2 exchange stages (partner q^2, then q^1) written exactly as the kernel writes
-them.  The forward leaves lane q holding X[k + E bitrev2(q)] at position 4k + q; the
-inverse undoes it.  Restating the kernel's butterflies here with numpy makes a sign,
-rotation or index mistake fail on the CPU before any GPU run."""
+L lanes (4, or 8 at n = 128) own one row or column; lane q holds points L j + q.  Per
+lane: an E = n/L point DFT over j and a twiddle W_n^(q k); then a DFT_L across the lanes.
+With 8 lanes that starts with a radix-2 exchange with lane q^4 and the W_8 twiddle on
+the upper lanes; every quad then does its DFT_4 as two radix-2 exchanges (partner q^2,
+then q^1) with per-lane signs and the lane-3 rotation, as the kernel writes them.  The
+forward leaves lane q holding X[k + E bitrev(q)] at position L k + q; the inverse undoes
+it.  Restating the kernel's butterflies here with numpy makes a sign, rotation or index
+mistake fail on the CPU before any GPU run."""
 import numpy
 import pytest
 
-from phaser_amd.optics import scr_to_freq
+from phaser_amd.optics import fft_lanes, scr_to_freq
 
 
 def _lane_fwd(lanes, n):
@@ -37,20 +39,62 @@ def _lane_inv(lanes, n):
     return [numpy.fft.ifft(x[q]) * E for q in range(4)]
 
 
+def _w8(q):
+    """W_8^(q-4) on the upper lanes of an 8-lane group, 1 on the lower ones."""
+    return numpy.exp(-2j * numpy.pi * (q - 4) / 8) if q >= 4 else 1.0
+
+
+def _quad_fwd(a):
+    y = []
+    for qq in range(4):
+        s1 = 1.0 if qq < 2 else -1.0
+        yy = s1 * a[qq] + a[qq ^ 2]
+        y.append(yy * -1j if qq == 3 else yy)
+    return [(-1.0 if qq & 1 else 1.0) * y[qq] + y[qq ^ 1] for qq in range(4)]
+
+
+def _quad_inv(z):
+    a = []
+    for qq in range(4):
+        aa = (-1.0 if qq & 1 else 1.0) * z[qq] + z[qq ^ 1]
+        a.append(aa * 1j if qq == 3 else aa)
+    return [(1.0 if qq < 2 else -1.0) * a[qq] + a[qq ^ 2] for qq in range(4)]
+
+
+def _lane8_fwd(lanes, n):
+    """8 lanes (n = 128): exchange with lane q^4 and the W_8 twiddle, then each quad's DFT-4."""
+    E = n // 8
+    tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
+    x = [numpy.fft.fft(lanes[q]) * tw[q * numpy.arange(E)] for q in range(8)]
+    a = [((1.0 if q < 4 else -1.0) * x[q] + x[q ^ 4]) * _w8(q) for q in range(8)]
+    return _quad_fwd(a[:4]) + _quad_fwd(a[4:])
+
+
+def _lane8_inv(lanes, n):
+    E = n // 8
+    tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
+    a = _quad_inv(lanes[:4]) + _quad_inv(lanes[4:])
+    a = [a[q] * numpy.conj(_w8(q)) for q in range(8)]
+    x = [(1.0 if q < 4 else -1.0) * a[q] + a[q ^ 4] for q in range(8)]
+    return [numpy.fft.ifft(x[q] * numpy.conj(tw[q * numpy.arange(E)])) * E for q in range(8)]
+
+
 def _axis_fwd(v, n):
-    """natural vector -> vector in scrambled positions (position 4k+q from lane q, k)."""
+    """natural vector -> vector in scrambled positions (position L k + q from lane q, k)."""
+    L = fft_lanes(n)
     out = numpy.empty(n, complex)
-    lanes = _lane_fwd([v[q::4] for q in range(4)], n)
-    for q in range(4):
-        out[q::4] = lanes[q]
+    lanes = (_lane_fwd if L == 4 else _lane8_fwd)([v[q::L] for q in range(L)], n)
+    for q in range(L):
+        out[q::L] = lanes[q]
     return out
 
 
 def _axis_inv(v, n):
+    L = fft_lanes(n)
     out = numpy.empty(n, complex)
-    lanes = _lane_inv([v[q::4] for q in range(4)], n)
-    for q in range(4):
-        out[q::4] = lanes[q]
+    lanes = (_lane_inv if L == 4 else _lane8_inv)([v[q::L] for q in range(L)], n)
+    for q in range(L):
+        out[q::L] = lanes[q]
     return out
 
 
@@ -81,6 +125,9 @@ def test_scrambled_order_small_case_by_hand():
     # n = 16, E = 4: lane 1 holds X[k + 8] (bitrev2(1) = 2), lane 2 holds X[k + 4]
     f = scr_to_freq(16)
     assert list(f[:8]) == [0, 8, 4, 12, 1, 9, 5, 13]
+    # n = 128, 8 lanes, E = 16: lane q holds X[k + 16 bitrev3(q)]
+    f = scr_to_freq(128)
+    assert list(f[:8]) == [0, 64, 32, 96, 16, 80, 48, 112] and f[8] == 1
 
 
 def test_pointwise_between_transforms_equals_fft_convolution():
