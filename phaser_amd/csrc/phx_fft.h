@@ -17,9 +17,9 @@
 // 4 waves per SIMD to hide each phase's latency instead of 2.
 //
 // A 2-D transform is: rows in registers -> transpose through LDS -> columns in registers.
-// The LDS plane has row stride n + 4 float2, which makes the column-phase accesses
-// (s[L j + q][c]) bank-conflict free, and the row-phase ones (s[r][L k + q]) as well for
-// L = 4 (2-way for L = 8).  A forward transform, a pointwise multiply in the (scrambled)
+// The LDS plane is padded (and for L = 8 XOR-swizzled, Plane::row_base / col_base) so that both the
+// row-phase accesses (s[r][L k + q]) and the column-phase ones (s[L j + q][c]) are
+// bank-conflict free.  A forward transform, a pointwise multiply in the (scrambled)
 // frequency domain and an inverse transform therefore cost two LDS round trips in total.
 //
 // Transforms are unnormalised: forward uses exp(-2 pi i nk/n), inverse exp(+2 pi i nk/n),
@@ -139,8 +139,21 @@ struct Plane {
   static constexpr int QL = PlaneLanes<N>::value;  // lanes per row / column
   static constexpr int E = N / QL;      // points per lane
   static constexpr int T = QL * N;      // threads per workgroup
-  static constexpr int LD = N + 4;      // LDS row stride (float2)
+  // LDS row stride (float2).  4 lanes: n + 4 makes both transpose directions bank-
+  // conflict free.  8 lanes: no plain stride does (one direction stays 2-way), so n + 8
+  // plus an XOR of column bit 2 with row bit 2 (checked exhaustively for every half-wave
+  // of both access patterns: 64 distinct banks each).
+  static constexpr int LD = QL == 8 ? N + 8 : N + 4;
   static constexpr int LDS_FLOAT2 = N * LD;
+  // element (row, col) lives at row * LD + (col ^ sw(row)); for the two transpose
+  // patterns the swizzle is a per-lane constant, so each is one base offset per lane
+  // plus compile-time steps
+  __device__ __forceinline__ int row_base() const {  // (row rc, col QL k + q): + QL k
+    return rc * LD + (QL == 8 ? (q ^ (((rc >> 2) & 1) << 2)) : q);
+  }
+  __device__ __forceinline__ int col_base() const {  // (row QL j + q, col rc): + QL LD j
+    return q * LD + (QL == 8 ? (rc ^ (((q >> 2) & 1) << 2)) : rc);
+  }
   float2* s;         // N x LD
   const float2* tw;  // W_N^e, e < N, in LDS
   int rc;            // row (row phase) or column (column phase) owned by this lane group
@@ -235,20 +248,20 @@ struct Plane {
   __device__ __forceinline__ void rows_to_cols(float2 (&x)[E]) {
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < E; ++k) s[rc * LD + QL * k + q] = x[k];
+    for (int k = 0; k < E; ++k) s[row_base() + QL * k] = x[k];
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < E; ++j) x[j] = s[(QL * j + q) * LD + rc];
+    for (int j = 0; j < E; ++j) x[j] = s[col_base() + QL * LD * j];
   }
   // column phase (x[j] at row QL j + q of column rc) -> row phase (x[k] at column QL k + q
   // of row rc)
   __device__ __forceinline__ void cols_to_rows(float2 (&x)[E]) {
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < E; ++j) s[(QL * j + q) * LD + rc] = x[j];
+    for (int j = 0; j < E; ++j) s[col_base() + QL * LD * j] = x[j];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < E; ++k) x[k] = s[rc * LD + QL * k + q];
+    for (int k = 0; k < E; ++k) x[k] = s[row_base() + QL * k];
   }
 
   // ----- composites -----
