@@ -122,6 +122,15 @@ def test_lsqml_multislice_modes_with_graphs():
     _check(*_run_all(state, pats, solver='lsqml', niter=5, grouping=48, use_graphs=True))
 
 
+@pytest.mark.parametrize('solver', ['lsqml', 'epie'])
+def test_n128_multislice_modes(solver):
+    """128^2 planes (the benchmark's size, whose transforms use 8 lanes per row): 3 slices,
+    2 modes, 2 iterations of the fused LSQ-ML sweeps / ePIE, against the oracle."""
+    _, state, pats = _problem(n=128, n_modes=2, n_slices=3, scan_shape=(4, 4), scan_step_px=6.0, jitter_px=0.5,
+                              seed=8)
+    _check(*_run_all(state, pats, solver=solver, niter=2, grouping=8, beta=0.5))
+
+
 def test_lsqml_anscombe_multislice():
     _, state, pats = _problem(n=32, n_modes=2, n_slices=2, scan_shape=(8, 8), scan_step_px=3.0, jitter_px=0.5,
                               seed=6, defocus=30.0)

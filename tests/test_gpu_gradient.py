@@ -57,6 +57,24 @@ def test_adjoint_matches_oracle_gradient(noise, recompute):
     assert e[0] < 1e-5 and e[1] < 1e-4 and e[2] < 1e-4 and e[3] < 1e-3
 
 
+def test_adjoint_n128_matches_oracle_gradient():
+    """The fused adjoint sweep on 128^2 planes (8 lanes per row) against the oracle."""
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import NOISE_MODELS
+    _cfg, state, pats = _problem(S=3, M=2, n=128, scan=(3, 3))
+    group = numpy.array(numpy.unravel_index(numpy.arange(9), (3, 3)))
+    model, _ = make_model(state, pats.patterns, pats.pattern_mask, NOISE_MODELS['amplitude'](), groups=[group])
+    model.forward(0)
+    model.backward(0)
+    loss = float(model.loss_tensor(0).item())
+    o_loss, o_obj, o_probe, o_pos = _oracle_grad(state, pats, group, 'amplitude')
+    e = (abs(loss - o_loss) / abs(o_loss), rel_err(model.ws.grad_obj.cpu().numpy(), o_obj),
+         rel_err(model.ws.grad_probe.view(model.ws.probe.shape).cpu().numpy(), o_probe),
+         rel_err(model.position_grads(0)[:9].cpu().numpy(), o_pos))
+    print(f"n=128: loss {e[0]:.2e} object {e[1]:.2e} probe {e[2]:.2e} positions {e[3]:.2e}", flush=True)
+    assert e[0] < 1e-5 and e[1] < 1e-4 and e[2] < 1e-4 and e[3] < 1e-3
+
+
 def test_recompute_equals_store():
     from phaser_amd.engines.gradient import make_model
     from phaser_amd.plan import AmplitudeNoisePlan
