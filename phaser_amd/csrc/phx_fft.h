@@ -7,20 +7,21 @@
 // n = E * L decomposition:
 //   1. in-register DFT_E over j (radix-2, compile-time twiddles),
 //   2. twiddle W_n^(q k1) (table in LDS),
-//   3. DFT_L across the lanes: for L = 8 a radix-2 exchange with lane q^4 plus the
-//      W_8 twiddle, then (for both) the DFT_4 of each quad as two radix-2 exchanges.
-//      All exchanges are DPP permutes (no LDS).
-// Lane q then holds X[k1 + E bitrev(q)], k1 < E, which it stores at "scrambled"
-// position p = L k1 + q: position p holds frequency f(p) = p/L + E bitrev(p % L).
-// The inverse runs the same steps backwards and returns natural order.  At n = 128 the
-// 8-lane layout gives 1024 threads per plane: with one plane per CU (its LDS), that is
-// 4 waves per SIMD to hide each phase's latency instead of 2.
+//   3. DFT_L across the lanes: for L = 4 the DFT_4 of each quad as two radix-2
+//      exchanges over DPP permutes; for L = 8 an exchange through LDS within the wave,
+//      then two in-register DFT_8s per lane.
+// The transform leaves each lane's results at "scrambled" positions p = L k + q
+// (register k of lane q); scr_to_freq gives the frequency held at p.  The inverse runs
+// the same steps backwards and returns natural order.  At n = 128 the 8-lane layout
+// gives 1024 threads per plane: with one plane per CU (its LDS), that is 4 waves per
+// SIMD to hide each phase's latency instead of 2.
 //
 // A 2-D transform is: rows in registers -> transpose through LDS -> columns in registers.
 // The LDS plane is padded (and for L = 8 XOR-swizzled, Plane::row_base / col_base) so that both the
 // row-phase accesses (s[r][L k + q]) and the column-phase ones (s[L j + q][c]) are
 // bank-conflict free.  A forward transform, a pointwise multiply in the (scrambled)
-// frequency domain and an inverse transform therefore cost two LDS round trips in total.
+// frequency domain and an inverse transform therefore cost two LDS round trips in total
+// (six at n = 128, with the in-wave exchanges).
 //
 // Transforms are unnormalised: forward uses exp(-2 pi i nk/n), inverse exp(+2 pi i nk/n),
 // so inverse(forward(a)) = n^2 a; callers scale by 1/n^2.  The (i)fftshift of the centred
@@ -107,15 +108,8 @@ __device__ __forceinline__ float dpp_xor1(float v) {
 __device__ __forceinline__ float dpp_xor2(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
 }
-// lane q ^ 4 within each group of 8: row_half_mirror (lane p <- 7 - p within 8) then
-// quad_perm [3,2,1,0] (p <- p ^ 3 within 4): 7 - (p ^ 3) = p ^ 4
-__device__ __forceinline__ float dpp_xor4(float v) {
-  const int m = __builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false);
-  return __int_as_float(__builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, false));
-}
 __device__ __forceinline__ float2 quad_xor1(float2 v) { return make_float2(dpp_xor1(v.x), dpp_xor1(v.y)); }
 __device__ __forceinline__ float2 quad_xor2(float2 v) { return make_float2(dpp_xor2(v.x), dpp_xor2(v.y)); }
-__device__ __forceinline__ float2 oct_xor4(float2 v) { return make_float2(dpp_xor4(v.x), dpp_xor4(v.y)); }
 
 // Lanes per row (row phase) or column (column phase): 8 at n = 128 (1024 threads: 4
 // waves per SIMD for the one plane a CU holds), 4 below.
@@ -124,14 +118,15 @@ struct PlaneLanes {
   static constexpr int value = N >= 128 ? 8 : 4;
 };
 
-// Frequency index held at scrambled position p of an N-point axis: with L lanes, lane q
-// ends holding X[k + (N/L) bitrev(q)] at position L k + q (bitrev over log2 L bits).
+// Frequency index held at scrambled position p = L k + q of an N-point axis (register k
+// of lane q after a forward transform).  4 lanes: X[k + (N/4) bitrev2(q)].  8 lanes
+// (N = 128): register k = 2 r + s of lane q holds X[2 q + s + 16 r].
 template <int N>
 __host__ __device__ __forceinline__ int scr_to_freq(int p) {
   constexpr int L = PlaneLanes<N>::value;
-  const int q = p % L;
-  const int br = L == 4 ? (((q & 1) << 1) | (q >> 1)) : (((q & 1) << 2) | (q & 2) | (q >> 2));
-  return p / L + (N / L) * br;
+  const int q = p % L, k = p / L;
+  if (L == 8) return 2 * q + (k & 1) + 16 * (k >> 1);
+  return k + (N / 4) * (((q & 1) << 1) | (q >> 1));
 }
 
 template <int N>
@@ -155,7 +150,7 @@ struct Plane {
     return q * LD + (QL == 8 ? (rc ^ (((q >> 2) & 1) << 2)) : rc);
   }
   float2* s;         // N x LD
-  const float2* tw;  // W_N^e, e < N, in LDS
+  const float2* tw;  // twiddles in LDS: tw[QL k + q] = W_N^(q k)
   int rc;            // row (row phase) or column (column phase) owned by this lane group
   int q;             // lane within the group
 
@@ -169,76 +164,100 @@ struct Plane {
   // the loop (dozens of registers live across all transforms).
   __device__ __forceinline__ void refresh() { __asm__ volatile("" : "+v"(rc), "+v"(q)); }
 
-  // ----- one 1-D transform along the lane group (registers + DPP only) -----
-  // After the in-lane DFT and the twiddle W_N^(q k), a DFT across the lanes:
-  //  * with 8 lanes, first a radix-2 exchange with lane q^4 (y = +-x + partner) and the
-  //    twiddle W_8^(q-4) on the upper four lanes;
-  //  * then, on each quad, the DFT-4 as two radix-2 exchanges (partner q^2, then q^1),
-  //    each one FMA per component with a per-lane sign (+-1 is exact), and the one
-  //    rotation of the radix-2x2 split (by -i on lane 3 of the quad in the forward, +i in
-  //    the inverse, applied before the exchange that reads it) as a select.
+  // ----- one 1-D transform along the lane group -----
+  // After the in-lane DFT_E and the twiddle W_N^(q k), the DFT across the L lanes:
+  //  * 4 lanes: two radix-2 exchanges over DPP quad_perm (partner q^2, then q^1), each
+  //    one FMA per component with a per-lane sign (+-1 is exact), and the one rotation
+  //    of the radix-2x2 split (by -i on lane 3 in the forward, +i in the inverse,
+  //    applied before the exchange that reads it) as a select;
+  //  * 8 lanes (N = 128, E = 16): an exchange through LDS inside the wave, after which
+  //    lane q holds, for k1 = 2q and 2q+1, the values of all 8 lanes, and two in-lane
+  //    DFT-8s finish the transform (fewer VALU instructions than three DPP stages).
   // No lane-dependent branches: a group's lanes run the same instructions.
-  __device__ __forceinline__ float2 w8_lane() const {
-    // W_8^(q-4) for the upper lanes of an 8-lane group, 1 for the lower ones
-    constexpr float c = 0.70710678118654752f;
-    const int r = q & 3;
-    const bool up = q >= 4;
-    const float wr = !up ? 1.f : (r == 0 ? 1.f : (r == 1 ? c : (r == 2 ? 0.f : -c)));
-    const float wi = !up ? 0.f : (r == 0 ? 0.f : (r == 2 ? -1.f : -c));
-    return make_float2(wr, wi);
+  //
+  // Exchange slots (8 lanes).  A wave uses the LDS rows of its own 8 row groups,
+  // 8 (rc >> 3) .. + 7 (1088 float2 from a multiple of 32): value (k1 = 2h + s, of lane
+  // qq) of lane group g = rc & 7 sits at 8 (g & 3) + 526 (g >> 2) + qq + 33 h + 263 s.
+  // One half-wave is 4 groups (same g >> 2); writing all qq for one k1, or reading all
+  // h for one (qq, s), its 32 lanes land on 8 (g & 3) + (qq or h) + const mod 32: 32
+  // distinct float2 banks.  Both forms are a per-lane base plus compile-time offsets.
+  // In the row phase no other wave touches these rows; in the column phase other waves
+  // read them during the transpose, so the forward column pass waits at a barrier first
+  // (an inverse column pass always follows a forward one or starts a kernel).
+  __device__ __forceinline__ int xbase() const {
+    const int g = rc & 7;
+    return (rc >> 3) * 8 * LD + 8 * (g & 3) + 526 * (g >> 2);
   }
+  template <bool COL>
   __device__ __forceinline__ void fwd(float2 (&x)[E]) {
     dft<E, false>(x);
-    // tw[0] = 1 exactly, so lane 0 multiplies by 1
+    // tw[q] = W^0 = 1 exactly, so k = 0 needs no multiply
+    const float2* twq = tw + q;
 #pragma unroll
-    for (int k = 1; k < E; ++k) x[k] = cmul(x[k], tw[q * k]);
-    if (QL == 8) {
-      const float s4 = q < 4 ? 1.f : -1.f;
-      const float2 w = w8_lane();
+    for (int k = 1; k < E; ++k) x[k] = cmul(x[k], twq[QL * k]);
+    if constexpr (QL == 8) {
+      if (COL) __syncthreads();
+      const int wb = xbase() + q, rb = xbase() + 33 * q;
+#pragma unroll
+      for (int k = 0; k < E; ++k) s[wb + 33 * (k >> 1) + 263 * (k & 1)] = x[k];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float2 y[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) y[t] = s[rb + t + 263 * h];
+        dft<8, false>(y);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) x[2 * r + h] = y[r];
+      }
+    } else {
+      const int qq = q & 3;
+      const float s1 = qq < 2 ? 1.f : -1.f;   // y = x + p (qq < 2), p - x (qq >= 2)
+      const float s2 = (qq & 1) ? -1.f : 1.f; // out = y + p (even), p - y (odd)
+      const bool r3 = qq == 3;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const float2 p = oct_xor4(x[k]);
-        x[k] = cmul(make_float2(fmaf(s4, x[k].x, p.x), fmaf(s4, x[k].y, p.y)), w);
+        const float2 p = quad_xor2(x[k]);
+        const float yr = fmaf(s1, x[k].x, p.x), yi = fmaf(s1, x[k].y, p.y);
+        const float2 y = make_float2(r3 ? yi : yr, r3 ? -yr : yi);  // * W4^1 = -i
+        const float2 p2 = quad_xor1(y);
+        x[k] = make_float2(fmaf(s2, y.x, p2.x), fmaf(s2, y.y, p2.y));
       }
-    }
-    const int qq = q & 3;
-    const float s1 = qq < 2 ? 1.f : -1.f;   // y = x + p (qq < 2), p - x (qq >= 2)
-    const float s2 = (qq & 1) ? -1.f : 1.f; // out = y + p (even), p - y (odd)
-    const bool r3 = qq == 3;
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const float2 p = quad_xor2(x[k]);
-      const float yr = fmaf(s1, x[k].x, p.x), yi = fmaf(s1, x[k].y, p.y);
-      const float2 y = make_float2(r3 ? yi : yr, r3 ? -yr : yi);  // * W4^1 = -i
-      const float2 p2 = quad_xor1(y);
-      x[k] = make_float2(fmaf(s2, y.x, p2.x), fmaf(s2, y.y, p2.y));
     }
   }
+  template <bool COL>
   __device__ __forceinline__ void inv(float2 (&x)[E]) {
-    const int qq = q & 3;
-    const float s1 = qq < 2 ? 1.f : -1.f;
-    const float s2 = (qq & 1) ? -1.f : 1.f;
-    const bool r3 = qq == 3;
+    if constexpr (QL == 8) {
+      const int wb = xbase() + 33 * q, rb = xbase() + q;
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const float2 p = quad_xor1(x[k]);
-      const float ar = fmaf(s2, x[k].x, p.x), ai = fmaf(s2, x[k].y, p.y);
-      const float2 a = make_float2(r3 ? -ai : ar, r3 ? ar : ai);  // * conj(W4^1) = +i
-      const float2 p2 = quad_xor2(a);
-      x[k] = make_float2(fmaf(s1, a.x, p2.x), fmaf(s1, a.y, p2.y));
-    }
-    if (QL == 8) {
-      const float s4 = q < 4 ? 1.f : -1.f;
-      const float2 w = w8_lane();
+      for (int h = 0; h < 2; ++h) {
+        float2 y[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) y[r] = x[2 * r + h];
+        dft<8, true>(y);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s[wb + t + 263 * h] = y[t];
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < E; ++k) x[k] = s[rb + 33 * (k >> 1) + 263 * (k & 1)];
+    } else {
+      const int qq = q & 3;
+      const float s1 = qq < 2 ? 1.f : -1.f;
+      const float s2 = (qq & 1) ? -1.f : 1.f;
+      const bool r3 = qq == 3;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const float2 a = cmulc(x[k], w);
-        const float2 p = oct_xor4(a);
-        x[k] = make_float2(fmaf(s4, a.x, p.x), fmaf(s4, a.y, p.y));
+        const float2 p = quad_xor1(x[k]);
+        const float ar = fmaf(s2, x[k].x, p.x), ai = fmaf(s2, x[k].y, p.y);
+        const float2 a = make_float2(r3 ? -ai : ar, r3 ? ar : ai);  // * conj(W4^1) = +i
+        const float2 p2 = quad_xor2(a);
+        x[k] = make_float2(fmaf(s1, a.x, p2.x), fmaf(s1, a.y, p2.y));
       }
     }
+    const float2* twq = tw + q;
 #pragma unroll
-    for (int k = 1; k < E; ++k) x[k] = cmulc(x[k], tw[q * k]);
+    for (int k = 1; k < E; ++k) x[k] = cmulc(x[k], twq[QL * k]);
     dft<E, true>(x);
   }
 
@@ -323,15 +342,15 @@ struct Plane {
   }
   // row-phase natural data -> column-phase scrambled spectrum (py = QL k + q, px = rc)
   __device__ __forceinline__ void forward_regs(float2 (&x)[E]) {
-    fwd(x);
+    fwd<false>(x);
     rows_to_cols(x);
-    fwd(x);
+    fwd<true>(x);
   }
   // column-phase scrambled spectrum -> row-phase natural data
   __device__ __forceinline__ void inverse_regs(float2 (&x)[E]) {
-    inv(x);
+    inv<true>(x);
     cols_to_rows(x);
-    inv(x);
+    inv<false>(x);
   }
   // forward transform of src; fn(py, px, v&) sees every scrambled spectrum value
   template <class Src, class Fn, class Pin>
@@ -393,10 +412,14 @@ struct Plane {
   }
 };
 
-// Load the twiddle table (n entries, host-computed in FP64) into LDS.
+// Load the twiddle table into LDS, arranged by step: entry QL k + q holds W_N^(q k)
+// (q < QL, k < E: QL E = N entries), so lane q reads its E factors at one base plus
+// compile-time offsets, and the QL lanes of a group read adjacent entries (no bank
+// conflicts).  tw_glob holds W_N^e, e < N (host-computed in FP64).
 template <int N>
 __device__ __forceinline__ void load_twiddles(float2* tw_lds, const float2* tw_glob) {
-  for (int i = threadIdx.x; i < N; i += Plane<N>::T) tw_lds[i] = tw_glob[i];
+  constexpr int QL = Plane<N>::QL;
+  for (int i = threadIdx.x; i < N; i += Plane<N>::T) tw_lds[i] = tw_glob[(i % QL) * (i / QL)];
 }
 
 }  // namespace phx

@@ -77,20 +77,19 @@ def fft_lanes(n: int) -> int:
 
 
 def scr_to_freq(n: int) -> numpy.ndarray:
-    """Frequency index held at each scrambled position of an n-point axis.
+    """Frequency index held at each scrambled position p = L k + q of an n-point axis
+    (register k of lane q after a forward transform).
 
-    With L lanes per row, lane q ends the transform holding X[k + (n/L) bitrev(q)] at
-    position L k + q (bitrev over log2(L) bits)."""
+    4 lanes: X[k + (n/4) bitrev2(q)].  8 lanes (n = 128): register k = 2 r + s of lane q
+    holds X[2 q + s + 16 r]."""
     if n not in FFT_SIZES:
         raise ValueError(f"unsupported plane side {n}; the FFT kernels support {FFT_SIZES}")
     L = fft_lanes(n)
     p = numpy.arange(n)
-    q = p % L
-    if L == 4:
-        br = ((q & 1) << 1) | (q >> 1)
-    else:
-        br = ((q & 1) << 2) | (q & 2) | (q >> 2)
-    return p // L + (n // L) * br
+    q, k = p % L, p // L
+    if L == 8:
+        return 2 * q + (k & 1) + 16 * (k >> 1)
+    return k + (n // 4) * (((q & 1) << 1) | (q >> 1))
 
 
 def scramble(a: numpy.ndarray) -> numpy.ndarray:

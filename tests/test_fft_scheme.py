@@ -2,11 +2,12 @@
 
 L lanes (4, or 8 at n = 128) own one row or column; lane q holds points L j + q.  Per
 lane: an E = n/L point DFT over j and a twiddle W_n^(q k); then a DFT_L across the lanes.
-With 8 lanes that starts with a radix-2 exchange with lane q^4 and the W_8 twiddle on
-the upper lanes; every quad then does its DFT_4 as two radix-2 exchanges (partner q^2,
-then q^1) with per-lane signs and the lane-3 rotation, as the kernel writes them.  The
-forward leaves lane q holding X[k + E bitrev(q)] at position L k + q; the inverse undoes
-it.  Restating the kernel's butterflies here with numpy makes a sign, rotation or index
+With 4 lanes every quad does its DFT_4 as two radix-2 exchanges (partner q^2, then q^1)
+with per-lane signs and the lane-3 rotation, as the kernel writes them; with 8 lanes an
+in-wave exchange gives lane q the values of all lanes for k1 = 2q, 2q+1 and two DFT-8s
+finish.  The
+forward leaves each lane's results at positions L k + q (scr_to_freq names the frequency
+at each); the inverse undoes it.  Restating the kernel's butterflies here with numpy makes a sign, rotation or index
 mistake fail on the CPU before any GPU run."""
 import numpy
 import pytest
@@ -39,44 +40,29 @@ def _lane_inv(lanes, n):
     return [numpy.fft.ifft(x[q]) * E for q in range(4)]
 
 
-def _w8(q):
-    """W_8^(q-4) on the upper lanes of an 8-lane group, 1 on the lower ones."""
-    return numpy.exp(-2j * numpy.pi * (q - 4) / 8) if q >= 4 else 1.0
-
-
-def _quad_fwd(a):
-    y = []
-    for qq in range(4):
-        s1 = 1.0 if qq < 2 else -1.0
-        yy = s1 * a[qq] + a[qq ^ 2]
-        y.append(yy * -1j if qq == 3 else yy)
-    return [(-1.0 if qq & 1 else 1.0) * y[qq] + y[qq ^ 1] for qq in range(4)]
-
-
-def _quad_inv(z):
-    a = []
-    for qq in range(4):
-        aa = (-1.0 if qq & 1 else 1.0) * z[qq] + z[qq ^ 1]
-        a.append(aa * 1j if qq == 3 else aa)
-    return [(1.0 if qq < 2 else -1.0) * a[qq] + a[qq ^ 2] for qq in range(4)]
-
-
 def _lane8_fwd(lanes, n):
-    """8 lanes (n = 128): exchange with lane q^4 and the W_8 twiddle, then each quad's DFT-4."""
+    """8 lanes (n = 128), as the kernel: per lane the DFT-16 and twiddle; the in-wave
+    exchange gives lane q the values of every lane for k1 = 2q + s (s = 0, 1); two
+    DFT-8s over the source lane leave X[2q + s + 16 r] in register 2r + s."""
     E = n // 8
     tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
-    x = [numpy.fft.fft(lanes[q]) * tw[q * numpy.arange(E)] for q in range(8)]
-    a = [((1.0 if q < 4 else -1.0) * x[q] + x[q ^ 4]) * _w8(q) for q in range(8)]
-    return _quad_fwd(a[:4]) + _quad_fwd(a[4:])
+    a = [numpy.fft.fft(lanes[q]) * tw[q * numpy.arange(E)] for q in range(8)]
+    out = []
+    for q in range(8):
+        reg = numpy.empty(E, complex)
+        for s in range(2):
+            y = numpy.fft.fft([a[t][2 * q + s] for t in range(8)])
+            reg[s::2] = y
+        out.append(reg)
+    return out
 
 
 def _lane8_inv(lanes, n):
     E = n // 8
     tw = numpy.exp(-2j * numpy.pi * numpy.arange(n) / n)
-    a = _quad_inv(lanes[:4]) + _quad_inv(lanes[4:])
-    a = [a[q] * numpy.conj(_w8(q)) for q in range(8)]
-    x = [(1.0 if q < 4 else -1.0) * a[q] + a[q ^ 4] for q in range(8)]
-    return [numpy.fft.ifft(x[q] * numpy.conj(tw[q * numpy.arange(E)])) * E for q in range(8)]
+    z = [[numpy.fft.ifft(lanes[q][s::2]) * 8 for s in range(2)] for q in range(8)]   # z[q][s][q'']
+    x = [numpy.array([z[k1 >> 1][k1 & 1][qq] for k1 in range(E)]) for qq in range(8)]
+    return [numpy.fft.ifft(x[qq] * numpy.conj(tw[qq * numpy.arange(E)])) * E for qq in range(8)]
 
 
 def _axis_fwd(v, n):
@@ -125,9 +111,9 @@ def test_scrambled_order_small_case_by_hand():
     # n = 16, E = 4: lane 1 holds X[k + 8] (bitrev2(1) = 2), lane 2 holds X[k + 4]
     f = scr_to_freq(16)
     assert list(f[:8]) == [0, 8, 4, 12, 1, 9, 5, 13]
-    # n = 128, 8 lanes, E = 16: lane q holds X[k + 16 bitrev3(q)]
+    # n = 128, 8 lanes, E = 16: register k = 2r + s of lane q holds X[2q + s + 16r]
     f = scr_to_freq(128)
-    assert list(f[:8]) == [0, 64, 32, 96, 16, 80, 48, 112] and f[8] == 1
+    assert list(f[:8]) == [0, 2, 4, 6, 8, 10, 12, 14] and f[8] == 1 and f[16] == 16 and f[127] == 127
 
 
 def test_pointwise_between_transforms_equals_fft_convolution():
