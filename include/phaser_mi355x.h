@@ -71,8 +71,9 @@ int phx_scale_c64(void* ptr, int64_t count, float scale, hipStream_t stream);
  *
  * Frequency-domain arrays (propagators, patterns, mask, k-grids, probe spectra) are kept
  * in the FFT kernels' "scrambled" order: along each axis of an n-point transform,
- * position p holds frequency (p / 4) + (n / 4) * bitrev2(p % 4), bitrev2 = {0, 2, 1, 3}
- * (n = 16, 32, 64, 128).  Real-space arrays are natural order.
+ * position p holds frequency (p / 4) + (n / 4) * bitrev2(p % 4), bitrev2 = {0, 2, 1, 3},
+ * for n = 16, 32, 64, and 2 (p % 8) + (p / 8) % 2 + 16 (p / 16) for n = 128 (phx_scramble
+ * converts).  Real-space arrays are natural order.
  * --------------------------------------------------------------------------------- */
 typedef struct phx_group_desc {
   int32_t n;          /* plane side, 16..128, power of two */
@@ -194,7 +195,8 @@ int phx_lsqml_object_update(const phx_group_desc* d, int slice, hipStream_t stre
 int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count, hipStream_t stream);
 /* d_probe[m] = sum_b farfield[b][m] (position order): the probe-update spectrum. */
 int phx_probe_spectrum_reduce(const phx_group_desc* d, hipStream_t stream);
-/* P += beta_P (sum_b alpha_P) ifft2(d_probe)/(gom + reg) * gom/(gom + 1e-16); d_probe = 0. */
+/* P += beta_P (sum_b alpha_P) ifft2(d_probe)/(gom + reg) * gom/(gom + 1e-16); d_probe = 0;
+ * then probe_k = fft2(P) of the updated probe (as phx_probe_spectra). */
 int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream);
 
 /* ePIE backward step at slice i (solvers.py:459-485), maxima first, then the update. */
@@ -232,12 +234,18 @@ int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, i
 int phx_farfield_gradient_update(const phx_group_desc* d, int poisson, hipStream_t stream);
 /* Fused adjoint sweep (gradient/run.py:293-349 through the hand-written adjoint of the
  * forward model): from the exit-wave gradient in chi_a, every slice in one launch with
- * the gradient carried in registers: grad_obj[i] += scatter(g conj(psi_i)), g <- g
- * conj(O_i), g <- ifft2(fft2(g) conj(prop_{i-1})); at slice 0 the probe-gradient
+ * the gradient carried in registers: with want_object the wave gradient g at slice i is
+ * stored as chi_i (chi_slices[i], chi_a at i = S-1) for phx_gradient_object_gather,
+ * g <- g conj(O_i), g <- ifft2(fft2(g) conj(prop_{i-1})); at slice 0 the probe-gradient
  * spectrum (d_probe) and the position gradients (red pos_num).  Needs every psi_i
- * stored (not the recompute mode). */
+ * stored (not the recompute mode) and, for the object gradient with S > 1, chi_slices. */
 int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe, int want_positions,
                        hipStream_t stream);
+/* grad_obj[i] = sum_{b,m} scatter(g_i conj(psi_i)) for slices first .. first+count-1
+ * from the wave gradients phx_gradient_sweep stored, gathered per object tile in group
+ * and mode order (no atomics; deterministic).  Writes every pixel of those slices (zero
+ * where no cutout covers it). */
+int phx_gradient_object_gather(const phx_group_desc* d, int first, int count, hipStream_t stream);
 /* grad_probe = ifft2(d_probe) (ortho); d_probe = 0. */
 int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream);
 /* Gradient-engine rescale factors into errors[b]: sum(D) / sum |psi_{S-1} O_{S-1}|^2

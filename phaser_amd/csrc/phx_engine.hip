@@ -578,9 +578,10 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
 // Fused adjoint sweep of the gradient engine (the reverse of group_forward_k; cf. the
 // per-slice gradient_backward_k): starting from the exit-wave gradient in chi_a (written
 // by the far-field gradient update), for i = S-1 .. 0 with the gradient carried in
-// registers: grad_O_i += scatter(g conj(psi_i)) (atomics), g <- g conj(O_i), and
+// registers: store g as chi_i (grad_O_i = sum_{b,m} scatter(g conj(psi_i)) is gathered
+// afterwards by gradient_object_gather_k, without atomics), g <- g conj(O_i), and
 // g <- ifft2(fft2(g) conj(p_{i-1})); at slice 0 the probe-gradient spectrum and the
-// position gradients.  Needs every psi_i stored.
+// position gradients.  Needs every psi_i stored and, with S > 1, chi_slices.
 template <int N, int T, bool WANT_OBJ, bool WANT_P0>
 __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want_probe, int want_pos) {
   const int m = blockIdx.x, b = blockIdx.y;
@@ -602,25 +603,22 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   }
   for (int i = S - 1; i >= 0; --i) {
     pl.refresh();
-    const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
     const float2* obj_i = (const float2*)d.object + i * HW;
-    float2* gO = (float2*)d.grad_obj + i * HW;
     const int yy = pl.rc;
+    // the wave gradient at slice i goes to chi_i (chi_a itself at i = S-1), for
+    // gradient_object_gather_k to form grad_O_i without atomics
+    float2* gi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
+    const bool store = WANT_OBJ && i < S - 1;
 #pragma unroll
     for (int j0 = 0; j0 < E; j0 += G) {
-      // all of a group's loads before its atomics (which the compiler will not reorder)
-      float2 ps[G], ob[G];
+      float2 ob[G];
 #pragma unroll
-      for (int u = 0; u < G; ++u) {
-        const int xx = pl.QL * (j0 + u) + pl.q;
-        if (WANT_OBJ) ps[u] = psi_i[yy * N + xx];
-        ob[u] = obj_at(obj_i, c, yy, xx);
-      }
+      for (int u = 0; u < G; ++u) ob[u] = obj_at(obj_i, c, yy, pl.QL * (j0 + u) + pl.q);
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int xx = pl.QL * (j0 + u) + pl.q;
         const float2 gv = cscale(x[j0 + u], inv_nn);
-        if (WANT_OBJ && c.inb(yy, xx)) atomic_add_c64(&gO[c.idx(yy, xx)], cmulc(gv, ps[u]));
+        if (store) gi[yy * N + xx] = gv;
         x[j0 + u] = cmulc(gv, ob[u]);
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -660,46 +658,29 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   }
 }
 
-// LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
-//   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
-//   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
-// One workgroup owns a 64 x 4 object tile of slice first + blockIdx.z (one pixel per
-// thread; a wave reads 512 contiguous bytes of a plane row, so the arbitrary offset of a
-// cutout inside its 128-byte lines costs 1.25x, not the 2x of 16-pixel rows).  Wave 0 lists the group's positions whose cutout meets the tile, in group
-// order (ballot + prefix count); then every thread gathers both sums from those
-// positions' chi_i and psi_i planes, positions in group order and modes in order: no
-// atomics, and the result does not depend on scheduling.  Pixels no cutout covers are
-// left untouched (their reference update is exactly zero).
 constexpr int UPD_TW = 64, UPD_TH = 4;
 constexpr int UPD_LIST = 256;  // positions listed per pass
-__global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int first) {
-  const int i = first + (int)blockIdx.z;
-  __shared__ float sA;
+
+// The gather shared by the LSQ-ML object update and the gradient engine's object
+// gradient: for the pixel (y, x) of this thread in the 64 x 4 tile at (ty0, tx0) of
+// slice i, dO = sum_{b,m} chi_i conj(psi_i) and (with WANT_G) g = sum_{b,m} |psi_i|^2
+// over the group's cutouts that cover it, positions in group order, modes in order.
+// Every thread of the block must call it (it synchronises the block).
+template <bool WANT_G>
+__device__ __forceinline__ void gather_tile(const phx_group_desc& d, int i, int ty0, int tx0, int y, int x,
+                                            float2& dO, float& g) {
   __shared__ int s_b[UPD_LIST], s_y0[UPD_LIST], s_x0[UPD_LIST];
   __shared__ int s_cnt;
-  const RedLayout L = red_layout(d.n_slices, d.bmax, d.n_modes);
   const int gs = *d.group_size;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (wave == 0) {
-    double s = 0.0;
-    for (int b = lane; b < gs; b += 64) s += d.red[L.num_obj + i * d.bmax + b];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) {
-      const double den = d.red[L.den_obj + i] + (double)d.params[P_GAMMA];
-      sA = (float)((double)d.params[P_BETA_OBJ] * (s / den));
-    }
-  }
-  const float reg = d.params[P_REG_OBJ];
-  const int n = d.n, NN = n * n, M = d.n_modes, H = d.obj_h, W = d.obj_w;
-  const int ty0 = blockIdx.y * UPD_TH, tx0 = blockIdx.x * UPD_TW;
-  const int y = ty0 + threadIdx.x / UPD_TW, x = tx0 + threadIdx.x % UPD_TW;
+  const int n = d.n, NN = n * n, M = d.n_modes;
   const float2* chi = chi_in_buf(d, i);
   const float2* psi = (const float2*)d.psi;
-  float2 dO = make_float2(0.f, 0.f);
-  float g = 0.f;
+  dO = make_float2(0.f, 0.f);
+  g = 0.f;
   for (int c0 = 0; c0 < gs; c0 += UPD_LIST) {
     const int c1 = min(gs, c0 + UPD_LIST);
-    __syncthreads();  // the previous list is consumed (and sA is visible)
+    __syncthreads();  // the previous list is consumed
     if (wave == 0) {
       int cnt = 0;
       for (int b0 = c0; b0 < c1; b0 += 64) {
@@ -746,20 +727,56 @@ __global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, i
         dOb = cadd(dOb, cmulc(c1v, p1));
         dOb = cadd(dOb, cmulc(c2v, p2));
         dOb = cadd(dOb, cmulc(c3v, p3));
-        gb += abs2f(p0);
-        gb += abs2f(p1);
-        gb += abs2f(p2);
-        gb += abs2f(p3);
+        if (WANT_G) {
+          gb += abs2f(p0);
+          gb += abs2f(p1);
+          gb += abs2f(p2);
+          gb += abs2f(p3);
+        }
       }
       for (; m < M; ++m) {
         const float2 cv = cb[m * NN], ps = pb[m * NN];
         dOb = cadd(dOb, cmulc(cv, ps));
-        gb += abs2f(ps);
+        if (WANT_G) gb += abs2f(ps);
       }
       dO = cadd(dO, cscale(dOb, w));
-      g += gb * w;
+      if (WANT_G) g += gb * w;
     }
   }
+}
+
+// LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
+//   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
+//   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
+// One workgroup owns a 64 x 4 object tile of slice first + blockIdx.z (one pixel per
+// thread; a wave reads 512 contiguous bytes of a plane row, so the arbitrary offset of a
+// cutout inside its 128-byte lines costs 1.25x, not the 2x of 16-pixel rows).  Wave 0
+// lists the group's positions whose cutout meets the tile, in group order (ballot +
+// prefix count); then every thread gathers both sums from those positions' chi_i and
+// psi_i planes (gather_tile): no atomics, and the result does not depend on scheduling.
+// Pixels no cutout covers are left untouched (their reference update is exactly zero).
+__global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int first) {
+  const int i = first + (int)blockIdx.z;
+  __shared__ float sA;
+  const RedLayout L = red_layout(d.n_slices, d.bmax, d.n_modes);
+  const int gs = *d.group_size;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave == 0) {
+    double s = 0.0;
+    for (int b = lane; b < gs; b += 64) s += d.red[L.num_obj + i * d.bmax + b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) {
+      const double den = d.red[L.den_obj + i] + (double)d.params[P_GAMMA];
+      sA = (float)((double)d.params[P_BETA_OBJ] * (s / den));
+    }
+  }
+  const float reg = d.params[P_REG_OBJ];
+  const int H = d.obj_h, W = d.obj_w;
+  const int ty0 = blockIdx.y * UPD_TH, tx0 = blockIdx.x * UPD_TW;
+  const int y = ty0 + threadIdx.x / UPD_TW, x = tx0 + threadIdx.x % UPD_TW;
+  float2 dO;
+  float g;
+  gather_tile<true>(d, i, ty0, tx0, y, x, dO, g);  // its barriers also publish sA
   if (y >= H || x >= W || (g == 0.f && dO.x == 0.f && dO.y == 0.f)) return;
   const float A = sA;
   float2* obj_i = (float2*)d.object + (size_t)i * H * W;
@@ -770,24 +787,54 @@ __global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, i
   if (!(isfinite(o.x) && isfinite(o.y))) atomicMin(d.finite_flag, d.group_index);
 }
 
+// Object gradient of slice first + blockIdx.z for the gradient engine:
+// grad_O_i = sum_{b,m} scatter(g_i conj(psi_i)) from the wave gradients g_i that
+// gradient_sweep_k stored as chi_i; tiles and order as lsqml_object_update_k.  Writes
+// every pixel of its tiles (zero where no cutout covers it).
+__global__ void __launch_bounds__(256) gradient_object_gather_k(phx_group_desc d, int first) {
+  const int i = first + (int)blockIdx.z;
+  const int H = d.obj_h, W = d.obj_w;
+  const int ty0 = blockIdx.y * UPD_TH, tx0 = blockIdx.x * UPD_TW;
+  const int y = ty0 + threadIdx.x / UPD_TW, x = tx0 + threadIdx.x % UPD_TW;
+  float2 dO;
+  float g;
+  gather_tile<false>(d, i, ty0, tx0, y, x, dO, g);
+  if (y >= H || x >= W) return;
+  ((float2*)d.grad_obj)[(size_t)i * H * W + (size_t)y * W + x] = dO;
+}
+
 // d_probe[m] = sum_b farfield[b][m]: the per-position probe-update spectra written by
-// the slice-0 backward step, summed over the group in position order.
+// the slice-0 backward step, summed over the group in position order.  Loads are issued
+// eight positions ahead of the (ordered) additions.
 __global__ void __launch_bounds__(256) probe_spectrum_reduce_k(phx_group_desc d) {
   const int NN = d.n * d.n, M = d.n_modes, m = blockIdx.y;
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= NN) return;
   const int gs = *d.group_size;
-  const float2* F = (const float2*)d.farfield;
+  const size_t stride = (size_t)M * NN;
+  const float2* F = (const float2*)d.farfield + (size_t)m * NN + e;
   float2 s = make_float2(0.f, 0.f);
-  for (int b = 0; b < gs; ++b) s = cadd(s, F[((size_t)b * M + m) * NN + e]);
+  int b = 0;
+  for (; b + 8 <= gs; b += 8) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = F[(size_t)(b + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = cadd(s, v[u]);
+  }
+  for (; b < gs; ++b) s = cadd(s, F[(size_t)b * stride]);
   ((float2*)d.d_probe)[(size_t)m * NN + e] = s;
 }
 
-// P += beta_P (sum_b alpha_P) ifft2(sum_b fft2(dP) conj(filt))/(gom+reg) * gom/(gom+eps).
+// P += beta_P (sum_b alpha_P) ifft2(sum_b fft2(dP) conj(filt))/(gom+reg) * gom/(gom+eps),
+// then probe_k = fft2(P) of the updated probe (what probe_spectra_k computes), so the
+// next group starts from current spectra.  The update's operands (gom, P) are all
+// loaded before the first store, not point by point.
 template <int N, int T>
 __global__ void __launch_bounds__(T) lsqml_probe_apply_k(phx_group_desc d) {
   PHX_PLANE_SETUP(N, T);
   constexpr int NN = N * N;
+  constexpr int E = Plane<N>::E;
   constexpr float inv_nn = 1.0f / (float)NN;
   __shared__ float sA;
   const int m = blockIdx.x, M = d.n_modes;
@@ -802,28 +849,41 @@ __global__ void __launch_bounds__(T) lsqml_probe_apply_k(phx_group_desc d) {
       sA = (float)((double)d.params[P_BETA_PROBE] * (s / den));
     }
   }
-  __syncthreads();
-  const float A = sA, reg = d.params[P_REG_PROBE];
   float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
   float2* P = (float2*)d.probe + (size_t)m * NN;
+  float2 x[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) x[k] = dPk[(pl.QL * k + pl.q) * N + pl.rc];
+#pragma unroll
+  for (int k = 0; k < E; ++k) dPk[(pl.QL * k + pl.q) * N + pl.rc] = make_float2(0.f, 0.f);
+  __syncthreads();  // sA
+  const float A = sA, reg = d.params[P_REG_PROBE];
+  pl.inverse_regs(x);
+  float2 pv[E];
+  float g[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int e = pl.rc * N + pl.QL * j + pl.q;
+    g[j] = d.gom[e];
+    pv[j] = P[e];
+  }
+  __builtin_amdgcn_sched_barrier(0);  // every load above issued before the first store
   bool bad = false;
-  pl.inverse(
-      [&](int py, int px) {
-        const int e = py * N + px;
-        const float2 v = dPk[e];
-        dPk[e] = make_float2(0.f, 0.f);
-        return v;
-      },
-      [&](int y, int x, float2 v) {
-        const int e = y * N + x;
-        const float g = d.gom[e];
-        const float2 avg = cscale(cscale(v, inv_nn), 1.0f / (g + reg));
-        const float2 u = cscale(cscale(cscale(avg, A), g), 1.0f / (g + 1e-16f));
-        const float2 o = cadd(P[e], u);
-        P[e] = o;
-        bad |= !(isfinite(o.x) && isfinite(o.y));
-      });
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const float2 avg = cscale(cscale(x[j], inv_nn), 1.0f / (g[j] + reg));
+    const float2 u = cscale(cscale(cscale(avg, A), g[j]), 1.0f / (g[j] + 1e-16f));
+    const float2 o = cadd(pv[j], u);
+    P[pl.rc * N + pl.QL * j + pl.q] = o;
+    bad |= !(isfinite(o.x) && isfinite(o.y));
+    x[j] = o;
+  }
   if (bad) atomicMin(d.finite_flag, d.group_index);
+  pl.refresh();  // no address arithmetic carried over from the update above
+  pl.forward_regs(x);
+  float2* Pk = (float2*)d.probe_k + (size_t)m * NN;
+#pragma unroll
+  for (int k = 0; k < E; ++k) Pk[(pl.QL * k + pl.q) * N + pl.rc] = x[k];
 }
 
 // ---------------------------------- ePIE ------------------------------------------
@@ -1366,7 +1426,7 @@ int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe,
                        hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (!d->chi_a || (d->n_slices > 1 && !d->props) || (want_object && !d->grad_obj) ||
+  if (!d->chi_a || (d->n_slices > 1 && !d->props) || (want_object && d->n_slices > 1 && !d->chi_slices) ||
       ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)))
     return PHX_EINVAL;
   if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // needs every psi_i
@@ -1401,6 +1461,19 @@ int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count
   const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH),
                   (unsigned)count);
   hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, first);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_gradient_object_gather(const phx_group_desc* d, int first, int count, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (first < 0 || count < 1 || first + count > d->n_slices || !d->chi_a || !d->grad_obj ||
+      (d->n_slices > 1 && !d->chi_slices) || (d->psi_slots > 0 && d->psi_slots < d->n_slices))
+    return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH),
+                  (unsigned)count);
+  hipLaunchKernelGGL(gradient_object_gather_k, grid, dim3(256), 0, stream, *d, first);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

@@ -236,7 +236,8 @@ class GroupWorkspace:
         c("phx_farfield", d, int(illum and self.S > 1), int(calc_error), st)
 
     def launch_lsqml_group(self, g, *, update_object, update_probe, update_position, calc_error):
-        """lsqml_run (solvers.py:171-290) for group slot g."""
+        """lsqml_run (solvers.py:171-290) for group slot g.  Groups are launched in order
+        0, 1, .. within an iteration (the fused path transforms the probe only at g = 0)."""
         d, st = self.desc_refs[g], stream_handle()
         c = self.call
         if self.fused and (self.chi_slices is not None or self.S == 1):
@@ -250,11 +251,15 @@ class GroupWorkspace:
             main = self.torch.cuda.current_stream()
             sh = ctypes.c_void_p(side.cuda_stream) if side is not None else st
             c("phx_group_begin", d, st)
-            c("phx_probe_spectra", d, st)
             if update_probe:
                 if side is not None:
                     side.wait_stream(main)
                 c("phx_group_obj_mag", d, sh)
+            if g == 0:
+                # later groups find probe_k current: the previous group's probe step
+                # writes fft2 of the probe it updates, and without a probe update the
+                # probe is the one group 0 transformed
+                c("phx_probe_spectra", d, st)
             c("phx_group_forward", d, st)
             c("phx_lsqml_backward_sweep", d, int(update_object), int(update_probe), int(calc_error), st)
             if update_position:

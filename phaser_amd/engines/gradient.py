@@ -55,8 +55,10 @@ class MultisliceModel:
         self.poisson = int(poisson)
         self.L = red_layout(ws.S, ws.B, ws.M)
         self._pending = None
-        # whole-sweep kernels need every slice's wave stored (not the recompute mode)
-        self.fused = (ws.psi_slots >= ws.S) if fused is None else (bool(fused) and ws.psi_slots >= ws.S)
+        # whole-sweep kernels need every slice's wave stored (not the recompute mode) and,
+        # with several slices, the per-slice wave-gradient planes (chi_slices)
+        ok = ws.psi_slots >= ws.S and (ws.S == 1 or ws.chi_slices is not None)
+        self.fused = ok if fused is None else (bool(fused) and ok)
 
     def dry_run(self, g):
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
@@ -67,12 +69,14 @@ class MultisliceModel:
             _lib.call("phx_forward_slice", d, i, 0, st)
         _lib.call("phx_dry_run_stats_gradient", d, st)
 
-    def forward(self, g):
+    def forward(self, g, probe_spectra=True):
         """Launch the forward model and the loss gradient w.r.t. the exit wave; the group
-        loss lands in ws.red[g, loss]."""
+        loss lands in ws.red[g, loss].  probe_spectra=False reuses the probe spectra of
+        the previous forward (valid while the probe is unchanged)."""
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         _lib.call("phx_group_begin", d, st)
-        _lib.call("phx_probe_spectra", d, st)
+        if probe_spectra or not self.fused:
+            _lib.call("phx_probe_spectra", d, st)
         if self.fused:
             # every slice in one launch, then the far-field gradient and loss
             _lib.call("phx_group_forward", d, st)
@@ -93,14 +97,18 @@ class MultisliceModel:
         if self._pending != g:
             raise RuntimeError("backward() must follow forward() of the same group")
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
-        if want_object:
-            _lib.call("phx_fill_u32", _p(ws.grad_obj), 0, ws.grad_obj.numel() * 2, st)
         if self.fused:
+            # the sweep stores each slice's wave gradient; one gather per object tile then
+            # forms every slice's object gradient (all pixels written, no atomics)
             _lib.call("phx_gradient_sweep", d, int(want_object), int(want_probe), int(want_positions), st)
+            if want_object:
+                _lib.call("phx_gradient_object_gather", d, 0, ws.S, st)
             if want_probe:
                 _lib.call("phx_gradient_probe_finish", d, st)
             self._pending = None
             return
+        if want_object:
+            _lib.call("phx_fill_u32", _p(ws.grad_obj), 0, ws.grad_obj.numel() * 2, st)
         recompute = ws.psi_slots < ws.S
         for i in range(ws.S - 1, -1, -1):
             if recompute and 1 <= i <= ws.S - 3:
@@ -126,7 +134,8 @@ def make_model(state, patterns, pattern_mask, noise, *, groups, bwlim_frac=2 / 3
     bmax = max(numpy.asarray(g).shape[-1] for g in groups)
     ws = GroupWorkspace(state, numpy.asarray(patterns), numpy.asarray(pattern_mask), props,
                         noise_offset=noise.offset, noise_eps=noise.eps, noise_var=1.0 + noise.gaussian_variance,
-                        n_groups=len(groups), bmax=bmax, psi_slots=3 if recompute else 0, gradients=True)
+                        n_groups=len(groups), bmax=bmax, psi_slots=3 if recompute else 0, gradients=True,
+                        keep_chi=not recompute)
     ws.set_positions(state.object.sampling, state.scan)
     ws.set_groups(list(groups))
     return MultisliceModel(ws, poisson=(noise.name == 'poisson')), state
@@ -323,7 +332,7 @@ def run_engine(args, props, *, recompute=False):
     B = int(numpy.ceil(n_pos / G))
     ws = GroupWorkspace(state, patterns, mask, props_arr, noise_offset=noise.offset, noise_eps=noise.eps,
                         noise_var=1.0 + noise.gaussian_variance, n_groups=G, bmax=B,
-                        psi_slots=3 if recompute else 0, gradients=True)
+                        psi_slots=3 if recompute else 0, gradients=True, keep_chi=not recompute)
     ws.set_positions(state.object.sampling, state.scan)
     model = MultisliceModel(ws, poisson=(noise.name == 'poisson'))
 
@@ -356,7 +365,8 @@ def run_engine(args, props, *, recompute=False):
         ws.reset_finite_flag()
         want_obj, want_probe, want_pos = ('object' in iter_vars), ('probe' in iter_vars), ('positions' in iter_vars)
         for g in range(G):
-            model.forward(g)
+            # the probe spectra change only with the probe
+            model.forward(g, probe_spectra=(g == 0 or want_probe))
             model.backward(g, want_object=want_obj, want_probe=want_probe, want_positions=want_pos)
             bsz = float(ws.sizes_host[g])
             loss_ptr = ctypes.c_void_p(ws.red[g].data_ptr() + 8 * model.L['loss'])
