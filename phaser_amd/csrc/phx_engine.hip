@@ -51,6 +51,10 @@ struct Cut {
     return ((unsigned)(y0 + y) < (unsigned)H) & ((unsigned)(x0 + x) < (unsigned)W);
   }
   __device__ __forceinline__ int idx(int y, int x) const { return (y0 + y) * W + (x0 + x); }
+  // the n x n cutout lies inside the object (uniform per plane): no clamping needed
+  __device__ __forceinline__ bool interior(int n) const {
+    return y0 >= 0 && x0 >= 0 && y0 + n <= H && x0 + n <= W;
+  }
 };
 
 __device__ __forceinline__ Cut cut_of(const phx_group_desc& d, int p) {
@@ -420,6 +424,7 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
   const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
+  const bool inside = c.interior(N);
   float nrm = 0.f;  // per-lane FP32 partial of sum |psi_0|^2, FP64 block sum
   auto no_pin = [] {};
 
@@ -434,12 +439,19 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
     float2* psi_i = (float2*)d.psi + psi_off(d, i, b, m);
     const float2* obj_i = (const float2*)d.object + i * HW;
     const bool first = i == 0;
-    pl.rows_apply(x, [&](int yy, int xx, float2 v) {
-      v = cscale(v, inv_nn);
-      psi_i[yy * N + xx] = v;
-      if (first) nrm += abs2f(v);
-      return cmul(v, obj_at(obj_i, c, yy, xx));
-    }, [&] { __asm__ volatile("" : "+v"(nrm)); });
+    // an interior cutout reads its row at one base plus immediate offsets
+    auto step = [&](auto interior_tag) __attribute__((always_inline)) {
+      constexpr bool IN = decltype(interior_tag)::value;
+      const float2* orow = obj_i + (size_t)(c.y0 + pl.rc) * c.W + c.x0;
+      pl.rows_apply(x, [&](int yy, int xx, float2 v) {
+        v = cscale(v, inv_nn);
+        psi_i[yy * N + xx] = v;
+        if (first) nrm += abs2f(v);
+        return cmul(v, IN ? orow[xx] : obj_at(obj_i, c, yy, xx));
+      }, [&] { __asm__ volatile("" : "+v"(nrm)); });
+    };
+    if (inside) step(std::true_type{});
+    else step(std::false_type{});
     if (i == S - 1) break;
     const float2* prop = (const float2*)d.props + (size_t)i * NN;
     pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); });
@@ -507,27 +519,34 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
 
   // one slice step on chi_i (row phase, unscaled): store it, the alpha_O sums, and
   // x <- dP = chi_i conj(O_i); at slice 0 also the alpha_P sums
-  auto slice_step = [&](int i, auto probe_tag) {
+  auto slice_step = [&](int i, auto probe_tag, auto interior_tag) __attribute__((always_inline)) {
     constexpr bool PS = decltype(probe_tag)::value;
+    constexpr bool IN = decltype(interior_tag)::value;
     float2* chi_i = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
     const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
     const float2* obj_i = (const float2*)d.object + i * HW;
+    const float2* orow = obj_i + (size_t)(c.y0 + pl.rc) * c.W + c.x0;  // interior cutouts
     float numO = 0.f, denO = 0.f, numP = 0.f, denP = 0.f;
     pl.rows_apply(x, [&](int yy, int xx, float2 v) {
       const float2 cv = cscale(v, inv_nn);
       chi_i[yy * N + xx] = cv;
+      const float a = abs2f(cv);
       if (UPD_OBJ) {
-        const float2 ps = psi_i[yy * N + xx];
-        const float2 t = cmul(cmulc(cv, ps), ps);  // (chi conj(psi)) psi
-        numO += cv.x * t.x + cv.y * t.y;
-        denO += abs2f(t);
+        // t = (chi conj(psi)) psi = chi |psi|^2: Re(conj(chi) t) = |chi|^2 |psi|^2,
+        // |t|^2 = |chi|^2 |psi|^4
+        const float p2 = abs2f(psi_i[yy * N + xx]);
+        const float ap = a * p2;
+        numO += ap;
+        denO = fmaf(ap, p2, denO);
       }
-      const float2 o = obj_at(obj_i, c, yy, xx);
+      const float2 o = IN ? orow[xx] : obj_at(obj_i, c, yy, xx);
       const float2 dP = cmulc(cv, o);  // chi conj(O)
       if (PS) {
-        const float2 t2 = cmul(dP, o);
-        numP += cv.x * t2.x + cv.y * t2.y;
-        denP += abs2f(t2);
+        // t2 = dP O = chi |O|^2, as above
+        const float o2 = abs2f(o);
+        const float ao = a * o2;
+        numP += ao;
+        denP = fmaf(ao, o2, denP);
       }
       return dP;
     }, [&] {
@@ -551,14 +570,16 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
       }
     }
   };
+  // (cutouts read through obj_at: an interior-only copy of the sweep would need more
+  // registers than the plane allows alongside the general one)
   for (int i = S - 1; i > 0; --i) {
     pl.refresh();
-    slice_step(i, std::false_type{});
+    slice_step(i, std::false_type{}, std::false_type{});
     const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
     pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); });
   }
   pl.refresh();
-  slice_step(0, std::integral_constant<bool, PROBE>{});
+  slice_step(0, std::integral_constant<bool, PROBE>{}, std::false_type{});
   if (PROBE) {
     // per-plane fft2(dP) conj(filt_b) into this plane's (already consumed) farfield slot
     const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
@@ -594,6 +615,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   const int M = d.n_modes, S = d.n_slices;
   const int p = d.group_pos[b];
   const Cut c = cut_of(d, p);
+  const bool inside = c.interior(N);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   float2 x[E];
   {
@@ -609,20 +631,29 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     // gradient_object_gather_k to form grad_O_i without atomics
     float2* gi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
     const bool store = WANT_OBJ && i < S - 1;
+    auto step = [&](auto interior_tag) __attribute__((always_inline)) {
+      constexpr bool IN = decltype(interior_tag)::value;
+      const float2* orow = obj_i + (size_t)(c.y0 + yy) * c.W + c.x0;  // interior cutouts
 #pragma unroll
-    for (int j0 = 0; j0 < E; j0 += G) {
-      float2 ob[G];
+      for (int j0 = 0; j0 < E; j0 += G) {
+        float2 ob[G];
 #pragma unroll
-      for (int u = 0; u < G; ++u) ob[u] = obj_at(obj_i, c, yy, pl.QL * (j0 + u) + pl.q);
+        for (int u = 0; u < G; ++u) {
+          const int xx = pl.QL * (j0 + u) + pl.q;
+          ob[u] = IN ? orow[xx] : obj_at(obj_i, c, yy, xx);
+        }
 #pragma unroll
-      for (int u = 0; u < G; ++u) {
-        const int xx = pl.QL * (j0 + u) + pl.q;
-        const float2 gv = cscale(x[j0 + u], inv_nn);
-        if (store) gi[yy * N + xx] = gv;
-        x[j0 + u] = cmulc(gv, ob[u]);
+        for (int u = 0; u < G; ++u) {
+          const int xx = pl.QL * (j0 + u) + pl.q;
+          const float2 gv = cscale(x[j0 + u], inv_nn);
+          if (store) gi[yy * N + xx] = gv;
+          x[j0 + u] = cmulc(gv, ob[u]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    };
+    if (inside) step(std::true_type{});
+    else step(std::false_type{});
     if (i == 0) break;
     const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
     pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); });

@@ -89,11 +89,13 @@ def test_recompute_equals_store():
         res.append((model.ws.grad_obj.cpu().numpy(), model.ws.grad_probe.cpu().numpy(),
                     model.position_grads(0).cpu().numpy(), model.ws.psi.shape[0]))
     assert res[0][3] == 6 and res[1][3] == 3
-    # store mode runs the fused adjoint sweep, recompute mode the per-slice kernels: the
-    # object and probe gradients agree to 1e-6; the position gradient, an FP32 sum with
-    # strong cancellation, to 1e-5 (both paths also match the FP64 oracle above)
-    for a, b, tol in zip(res[0][:3], res[1][:3], (1e-6, 1e-6, 1e-5)):
-        assert rel_err(a, b) < tol
+    # store mode runs the fused adjoint sweep and gathers the object gradient per tile,
+    # recompute mode the per-slice kernels with atomics: different FP32 operation orders,
+    # agreeing to ~1e-6 (both paths also match the FP64 oracle above to 1e-4)
+    for a, b in zip(res[0][:3], res[1][:3]):
+        e = rel_err(a, b)
+        print(f"store vs recompute: {e:.2e}", flush=True)
+        assert e < 1e-5
 
 
 def test_autograd_function_backward():
