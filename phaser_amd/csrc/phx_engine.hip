@@ -575,8 +575,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   for (int i = S - 1; i > 0; --i) {
     pl.refresh();
     slice_step(i, std::false_type{}, std::false_type{});
-    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
-    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); });
+    pl.template convolve_prefetched<true>(x, (const float2*)d.props + (size_t)(i - 1) * NN);
   }
   pl.refresh();
   slice_step(0, std::integral_constant<bool, PROBE>{}, std::false_type{});

@@ -399,6 +399,20 @@ struct Plane {
     __builtin_amdgcn_sched_barrier(0);
     inverse_regs(x);
   }
+  // in-register forward -> product with the scrambled-order plane mul (conjugated with
+  // CONJ) -> inverse, the multiplier's loads issued before the forward transform so
+  // their latency hides behind it
+  template <bool CONJ>
+  __device__ __forceinline__ void convolve_prefetched(float2 (&x)[E], const float2* mul) {
+    float2 mv[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) mv[k] = mul[(QL * k + q) * N + rc];
+    forward_regs(x);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < E; ++k) x[k] = CONJ ? cmulc(x[k], mv[k]) : cmul(x[k], mv[k]);
+    inverse_regs(x);
+  }
   template <class Src, class Mul, class Dst, class Pin>
   __device__ __forceinline__ void convolve(Src src, Mul mul, Dst dst, Pin pin) {
     float2 x[E];
