@@ -265,7 +265,12 @@ struct Plane {
   // row phase (x[k] at scrambled column QL k + q of row rc) -> column phase (x[j] at row
   // QL j + q of column rc)
   __device__ __forceinline__ void rows_to_cols(float2 (&x)[E]) {
-    __syncthreads();
+    // The write goes to this wave's own rows.  With 8 lanes, every access of them by
+    // another wave (a transpose's column reads or writes) is followed by a barrier before
+    // the owner touches them again (the forward column pass's, or the one inside
+    // cols_to_rows), so none is needed here; with 4 lanes the column reads of a previous
+    // forward transform may still be in flight.
+    if (QL != 8) __syncthreads();
 #pragma unroll
     for (int k = 0; k < E; ++k) s[row_base() + QL * k] = x[k];
     __syncthreads();

@@ -268,6 +268,11 @@ class GroupWorkspace:
                 if side is not None:
                     side.wait_stream(main)
                 c("phx_probe_spectrum_reduce", d, sh)
+                if side is not None and update_object:
+                    # the object update starts after the reduction, so the probe step (four
+                    # workgroups that each need a whole CU) is dispatched before the update
+                    # fills every CU
+                    main.wait_stream(side)
                 c("phx_lsqml_probe_apply", d, sh)
             if update_object:
                 # after the position step, which reads the pre-update slice-0 cutout
