@@ -22,6 +22,7 @@ import numpy
 
 METRIC = "patterns·iter/s; time to NRMSE 0.05, learned-init vs plain, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3
 
 
 def log(msg):
@@ -65,6 +66,37 @@ def lsqml_bytes_per_pattern_iter(S, M, NN, HW, B):
     c, r = 8, 4
     return (2 * S * M * c * NN + 2 * S * r * NN + 2 * S * c * NN + r * NN + M * c * NN
             + (3 * S - 1) * M * c * NN + 2 * S * c * NN + S * HW * (3 * c + r) / B)
+
+
+def fft_flops(nn):
+    """5 N log2 N for one complex 2-D transform of N = nn points."""
+    return 5.0 * nn * numpy.log2(nn)
+
+
+def fft_microbench(torch, n=128, batch=2048, reps=5):
+    """Batched forward fft2 (phx_fft2_c64) of `batch` n x n complex64 planes: average
+    launch time over `reps` launches (HIP events on the launch stream)."""
+    import ctypes
+    from phaser_amd import _lib
+    from phaser_amd.optics import twiddles
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    a = torch.randn((batch, n, n), dtype=torch.complex64, device='cuda')
+    out = torch.empty_like(a)
+    tw = torch.from_numpy(twiddles(n)).cuda()
+    args = (ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(out.data_ptr()), batch, n, 0,
+            ctypes.c_void_p(tw.data_ptr()), st)
+    _lib.call("phx_fft2_c64", *args)   # warm-up
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        _lib.call("phx_fft2_c64", *args)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    tflops = batch * fft_flops(n * n) / (us * 1e-6) / 1e12
+    gbs = 2 * a.numel() * 8 / (us * 1e-6) / 1e9
+    return {"n": n, "batch": batch, "avg_launch_us": us, "tflops": tflops,
+            "frac_fp32_peak": tflops / FP32_VECTOR_PEAK_TFLOPS, "gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
 
 
 def time_kernels(ws, groups, torch, n_groups):
@@ -222,6 +254,13 @@ def main():
             traffic = json.load(f)['kernels'].get(dom, {}).get('traffic_bytes')
     whole = per_pi * value / 1e9
     log(f"kernels: {json.dumps(kt)}")
+    # the second bound: FP32 vector FFT rate (4 M S transforms per pattern-iteration)
+    fft_tf = 4 * M * S * fft_flops(NN) * value / 1e12
+    fft = {"whole_iteration_tflops": fft_tf, "peak_tflops": FP32_VECTOR_PEAK_TFLOPS,
+           "whole_iteration_frac": fft_tf / FP32_VECTOR_PEAK_TFLOPS,
+           "transforms_per_pattern_iter": 4 * M * S,
+           "batched_fft2": fft_microbench(torch, n=int(round(NN ** 0.5)))}
+    log(f"fft rate: {json.dumps(fft)}")
 
     cpu = None
     if not args.no_cpu:
@@ -259,6 +298,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg, "avg_launch_us": kt[dom]['avg_us'],
                      "whole_iteration_alg_GBs": whole, "whole_iteration_frac": whole / HBM_PEAK_GBS},
+        "fft_rate": fft,
         "cpu_baseline": cpu,
         "detector_error_first_last": [errs[0], errs[-1]],
         "time_to_nrmse": ttn,

@@ -50,7 +50,7 @@ int phx_fill_u32(void* ptr, uint32_t value, int64_t count, hipStream_t stream);
 
 /* Batched fft2 (inverse=0) / ifft2 (inverse=1) with phaser's convention
  * (phaser/utils/num.py:434-468: centred real space, corner-origin reciprocal space,
- * norm='ortho') over `batch` n x n planes, n in {16, 32, 64, 128}.  `twiddle` holds the
+ * norm='ortho') over `batch` n x n planes, n in {16, 32, 64, 128, 192}.  `twiddle` holds the
  * n complex64 values exp(-2 pi i j / n).  in != out. */
 int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int inverse, const void* twiddle,
                  hipStream_t stream);
@@ -72,11 +72,11 @@ int phx_scale_c64(void* ptr, int64_t count, float scale, hipStream_t stream);
  * Frequency-domain arrays (propagators, patterns, mask, k-grids, probe spectra) are kept
  * in the FFT kernels' "scrambled" order: along each axis of an n-point transform,
  * position p holds frequency (p / 4) + (n / 4) * bitrev2(p % 4), bitrev2 = {0, 2, 1, 3},
- * for n = 16, 32, 64, and 2 (p % 8) + (p / 8) % 2 + 16 (p / 16) for n = 128 (phx_scramble
- * converts).  Real-space arrays are natural order.
+ * for n = 16, 32, 64, 192 (with n / 4 = 48), and 2 (p % 8) + (p / 8) % 2 + 16 (p / 16) for
+ * n = 128 (phx_scramble converts).  Real-space arrays are natural order.
  * --------------------------------------------------------------------------------- */
 typedef struct phx_group_desc {
-  int32_t n;          /* plane side, 16..128, power of two */
+  int32_t n;          /* plane side: 16, 32, 64, 128 or 192 */
   int32_t n_modes;    /* probe modes M */
   int32_t n_slices;   /* object slices S */
   int32_t obj_h;      /* object slice height H */
@@ -120,6 +120,8 @@ typedef struct phx_group_desc {
   void* chi_slices;          /* (S-1, bmax, M, n, n) complex64 or NULL.  When set, the
                                 backward steps keep every slice's chi (chi_{S-1} stays in
                                 chi_a) so all object updates of a group can run at once */
+  void* fft_scratch;         /* (M, bmax, n, n) complex64, required for n = 192: planes larger
+                                than LDS are transposed through one plane per workgroup */
 } phx_group_desc;
 
 /* Doubles needed by phx_group_desc.red. */

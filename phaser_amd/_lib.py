@@ -32,6 +32,7 @@ class GroupDesc(ctypes.Structure):
         ("red", c_vp), ("errors", c_vp), ("aux", c_vp), ("finite_flag", c_vp),
         ("psi_slots", c_i32), ("pad_", c_i32), ("grad_obj", c_vp), ("grad_probe", c_vp),
         ("chi_slices", c_vp),
+        ("fft_scratch", c_vp),
     ]
 
 

@@ -74,30 +74,39 @@ template <int N, int T>
 __global__ void __launch_bounds__(T) fft2_conv_k(const float2* __restrict__ in, float2* __restrict__ out,
                                                  const float2* __restrict__ twg, int inverse) {
   static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");
+  constexpr int E = Plane<N>::E;
   __shared__ float2 lds[Plane<N>::LDS_FLOAT2 + N];
-  Plane<N> pl(lds, lds + Plane<N>::LDS_FLOAT2);
-  load_twiddles<N>(lds + Plane<N>::LDS_FLOAT2, twg);
-  __syncthreads();
   const float2* a = in + (size_t)blockIdx.x * N * N;
   float2* b = out + (size_t)blockIdx.x * N * N;
+  // a plane too large for LDS is transposed through its own output plane, which is
+  // written only after the transform (behind a barrier)
+  Plane<N> pl(Plane<N>::IN_LDS ? lds : b, lds + Plane<N>::LDS_FLOAT2);
+  load_twiddles<N>(lds + Plane<N>::LDS_FLOAT2, twg);
+  __syncthreads();
   const float sc = 1.0f / (float)N;
+  float2 x[E];
   if (!inverse) {
     // fft2(a) = (-1)^(ky+kx) * FFT(a) / N  for even N
-    pl.forward([&](int y, int x) { return a[y * N + x]; },
-               [&](int py, int px, float2& v) {
-                 const int ky = scr_to_freq<N>(py), kx = scr_to_freq<N>(px);
-                 const float sg = ((ky + kx) & 1) ? -sc : sc;
-                 b[ky * N + kx] = cscale(v, sg);
-               });
+    pl.load_rows([&](int y, int xx) { return a[y * N + xx]; }, x);
+    pl.forward_regs(x);
+    if (!Plane<N>::IN_LDS) __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int ky = scr_to_freq<N>(pl.QL * k + pl.q), kx = scr_to_freq<N>(pl.rc);
+      const float sg = ((ky + kx) & 1) ? -sc : sc;
+      b[ky * N + kx] = cscale(x[k], sg);
+    }
   } else {
     // ifft2(A) = IFFT((-1)^(ky+kx) * A) / N
-    pl.inverse(
-        [&](int py, int px) {
-          const int ky = scr_to_freq<N>(py), kx = scr_to_freq<N>(px);
-          const float sg = ((ky + kx) & 1) ? -sc : sc;
-          return cscale(a[ky * N + kx], sg);
-        },
-        [&](int y, int x, float2 v) { b[y * N + x] = v; });
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int ky = scr_to_freq<N>(pl.QL * k + pl.q), kx = scr_to_freq<N>(pl.rc);
+      const float sg = ((ky + kx) & 1) ? -sc : sc;
+      x[k] = cscale(a[ky * N + kx], sg);
+    }
+    pl.inverse_regs(x);
+    if (!Plane<N>::IN_LDS) __syncthreads();
+    pl.store_rows([&](int y, int xx, float2 v) { b[y * N + xx] = v; }, x);
   }
 }
 
@@ -204,6 +213,7 @@ int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int invers
     case 32: return launch_fft2<32, 128>(in, out, batch, inverse, twiddle, stream);
     case 64: return launch_fft2<64, 256>(in, out, batch, inverse, twiddle, stream);
     case 128: return launch_fft2<128, 1024>(in, out, batch, inverse, twiddle, stream);
+    case 192: return launch_fft2<192, 768>(in, out, batch, inverse, twiddle, stream);
     default: return PHX_EUNSUPPORTED;
   }
 }
@@ -217,6 +227,7 @@ int phx_scramble(const void* in, void* out, int64_t batch, int64_t n, int is_com
     case 32: return launch_scramble<32>(in, out, batch, is_complex, unscramble, stream);
     case 64: return launch_scramble<64>(in, out, batch, is_complex, unscramble, stream);
     case 128: return launch_scramble<128>(in, out, batch, is_complex, unscramble, stream);
+    case 192: return launch_scramble<192>(in, out, batch, is_complex, unscramble, stream);
     default: return PHX_EUNSUPPORTED;
   }
 }

@@ -1,11 +1,11 @@
-// Batched 2-D complex FFT of n x n planes (n = 16, 32, 64, 128) for gfx950, one plane
-// per workgroup of L n threads.
+// Batched 2-D complex FFT of n x n planes (n = 16, 32, 64, 128, 192) for gfx950, one
+// plane per workgroup of L n threads.
 //
-// Layout.  A group of L lanes (L = 4 for n <= 64, 8 for n = 128) owns one row (row
+// Layout.  A group of L lanes (L = 8 for n = 128, 4 otherwise) owns one row (row
 // phase) or one column (column phase) of the plane; lane q holds the E = n/L points
 // with index L j + q, j < E.  A length-n transform along the group is a four-step
 // n = E * L decomposition:
-//   1. in-register DFT_E over j (radix-2, compile-time twiddles),
+//   1. in-register DFT_E over j (radix-2, compile-time twiddles; E = 48 as 3 x 16),
 //   2. twiddle W_n^(q k1) (table in LDS),
 //   3. DFT_L across the lanes: for L = 4 the DFT_4 of each quad as two radix-2
 //      exchanges over DPP permutes; for L = 8 an exchange through LDS within the wave,
@@ -16,7 +16,8 @@
 // gives 1024 threads per plane: with one plane per CU (its LDS), that is 4 waves per
 // SIMD to hide each phase's latency instead of 2.
 //
-// A 2-D transform is: rows in registers -> transpose through LDS -> columns in registers.
+// A 2-D transform is: rows in registers -> transpose through LDS (at n = 192, through a
+// global scratch plane of the workgroup's own) -> columns in registers.
 // The LDS plane is padded (and for L = 8 XOR-swizzled, Plane::row_base / col_base) so that both the
 // row-phase accesses (s[r][L k + q]) and the column-phase ones (s[L j + q][c]) are
 // bank-conflict free.  A forward transform, a pointwise multiply in the (scrambled)
@@ -68,9 +69,74 @@ template <> struct Log2<8> { static constexpr int v = 3; };
 template <> struct Log2<16> { static constexpr int v = 4; };
 template <> struct Log2<32> { static constexpr int v = 5; };
 
-// In-register length-R DFT (R = 2..32), natural order in and out.
+// exp(-2 pi i e / 48), e in [0, 31) (the twiddles of the 3 x 16 split below)
+__device__ __forceinline__ float2 w48(int e) {
+  constexpr float c[31] = {
+      1.f, 0.99144486137381038f, 0.96592582628906831f, 0.92387953251128674f, 0.86602540378443865f,
+      0.79335334029123517f, 0.70710678118654752f, 0.60876142900872066f, 0.5f, 0.38268343236508977f,
+      0.25881904510252074f, 0.13052619222005159f, 0.f, -0.13052619222005159f, -0.25881904510252074f,
+      -0.38268343236508977f, -0.5f, -0.60876142900872066f, -0.70710678118654752f, -0.79335334029123517f,
+      -0.86602540378443865f, -0.92387953251128674f, -0.96592582628906831f, -0.99144486137381038f, -1.f,
+      -0.99144486137381038f, -0.96592582628906831f, -0.92387953251128674f, -0.86602540378443865f,
+      -0.79335334029123517f, -0.70710678118654752f};
+  constexpr float s[31] = {
+      0.f, -0.13052619222005159f, -0.25881904510252074f, -0.38268343236508977f, -0.5f,
+      -0.60876142900872066f, -0.70710678118654752f, -0.79335334029123517f, -0.86602540378443865f,
+      -0.92387953251128674f, -0.96592582628906831f, -0.99144486137381038f, -1.f, -0.99144486137381038f,
+      -0.96592582628906831f, -0.92387953251128674f, -0.86602540378443865f, -0.79335334029123517f,
+      -0.70710678118654752f, -0.60876142900872066f, -0.5f, -0.38268343236508977f, -0.25881904510252074f,
+      -0.13052619222005159f, 0.f, 0.13052619222005159f, 0.25881904510252074f, 0.38268343236508977f, 0.5f,
+      0.60876142900872066f, 0.70710678118654752f};
+  return make_float2(c[e], s[e]);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 (&x)[R]);
+
+// In-register length-48 DFT (natural order in and out): 48 = 3 x 16, decimation in
+// time: X[k + 16 k2] = sum_b W_3^(b k2) W_48^(b k) Y_b[k], Y_b = DFT_16 of x[3 a + b].
+template <bool INV>
+__device__ __forceinline__ void dft48(float2 (&x)[48]) {
+  float2 y[3][16];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 16; ++a) y[b][a] = x[3 * a + b];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) dft<16, INV>(y[b]);
+#pragma unroll
+  for (int b = 1; b < 3; ++b)
+#pragma unroll
+    for (int k = 1; k < 16; ++k) y[b][k] = INV ? cmulc(y[b][k], w48(b * k)) : cmul(y[b][k], w48(b * k));
+  constexpr float h = 0.86602540378443865f;  // sqrt(3) / 2
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    // DFT-3: X0 = a + s, X1,2 = (a - s/2) -+ i h d (forward; +- in the inverse)
+    const float2 a = y[0][k], sm = cadd(y[1][k], y[2][k]), d = csub(y[1][k], y[2][k]);
+    const float2 t = make_float2(fmaf(-0.5f, sm.x, a.x), fmaf(-0.5f, sm.y, a.y));
+    const float2 r = INV ? make_float2(-h * d.y, h * d.x) : make_float2(h * d.y, -h * d.x);
+    x[k] = cadd(a, sm);
+    x[k + 16] = cadd(t, r);
+    x[k + 32] = csub(t, r);
+  }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_pow2(float2 (&x)[R]);
+
+// In-register length-R DFT (R = 2..32 or 48), natural order in and out.
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2 (&x)[R]) {
+  if constexpr (R == 48) {
+    dft48<INV>(x);
+  } else {
+    dft_pow2<R, INV>(x);
+  }
+}
+
+// radix-2, R = 2..32
+template <int R, bool INV>
+__device__ __forceinline__ void dft_pow2(float2 (&x)[R]) {
   constexpr int LG = Log2<R>::v;
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -112,10 +178,11 @@ __device__ __forceinline__ float2 quad_xor1(float2 v) { return make_float2(dpp_x
 __device__ __forceinline__ float2 quad_xor2(float2 v) { return make_float2(dpp_xor2(v.x), dpp_xor2(v.y)); }
 
 // Lanes per row (row phase) or column (column phase): 8 at n = 128 (1024 threads: 4
-// waves per SIMD for the one plane a CU holds), 4 below.
+// waves per SIMD for the one plane a CU holds), 4 otherwise (n = 192: 768 threads of 48
+// points each).
 template <int N>
 struct PlaneLanes {
-  static constexpr int value = N >= 128 ? 8 : 4;
+  static constexpr int value = N == 128 ? 8 : 4;
 };
 
 // Frequency index held at scrambled position p = L k + q of an N-point axis (register k
@@ -138,8 +205,12 @@ struct Plane {
   // conflict free.  8 lanes: no plain stride does (one direction stays 2-way), so n + 8
   // plus an XOR of column bit 2 with row bit 2 (checked exhaustively for every half-wave
   // of both access patterns: 64 distinct banks each).
-  static constexpr int LD = QL == 8 ? N + 8 : N + 4;
-  static constexpr int LDS_FLOAT2 = N * LD;
+  // A plane of up to 128 x 128 is transposed through LDS; a larger one (192 x 192:
+  // 288 KB) through a global scratch plane of the workgroup's own (L2-resident), with
+  // the same barriers.
+  static constexpr bool IN_LDS = N <= 128;
+  static constexpr int LD = !IN_LDS ? N : (QL == 8 ? N + 8 : N + 4);
+  static constexpr int LDS_FLOAT2 = IN_LDS ? N * LD : 0;
   // element (row, col) lives at row * LD + (col ^ sw(row)); for the two transpose
   // patterns the swizzle is a per-lane constant, so each is one base offset per lane
   // plus compile-time steps
@@ -149,7 +220,7 @@ struct Plane {
   __device__ __forceinline__ int col_base() const {  // (row QL j + q, col rc): + QL LD j
     return q * LD + (QL == 8 ? (rc ^ (((q >> 2) & 1) << 2)) : rc);
   }
-  float2* s;         // N x LD
+  float2* s;         // N x LD, in LDS or (IN_LDS false) global memory
   const float2* tw;  // twiddles in LDS: tw[QL k + q] = W_N^(q k)
   int rc;            // row (row phase) or column (column phase) owned by this lane group
   int q;             // lane within the group

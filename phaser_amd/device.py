@@ -69,8 +69,9 @@ class GroupWorkspace:
         assert obj.is_contiguous() and probe.is_contiguous()
         self.S, self.H, self.W = (int(v) for v in obj.shape)
         self.M, ny, nx = (int(v) for v in probe.shape)
-        if ny != nx or ny not in (16, 32, 64, 128):
-            raise NotImplementedError(f"probe shape {ny}x{nx}: the LDS kernels support square 16..128 planes")
+        if ny != nx or ny not in (16, 32, 64, 128, 192):
+            raise NotImplementedError(f"probe shape {ny}x{nx}: the plane kernels support square planes of side "
+                                      "16, 32, 64, 128 or 192")
         self.n = n = ny
         self.NN = n * n
         pats = numpy.asarray(patterns)
@@ -128,6 +129,8 @@ class GroupWorkspace:
         # LSQ-ML keeps every slice's chi so one launch updates all slices of a group
         self.chi_slices = (torch.empty((S - 1, B, M, NN), dtype=c64, device=dev)
                            if keep_chi and S > 1 else None)
+        # planes too large for LDS are transposed through one scratch plane per workgroup
+        self.fft_scratch = torch.empty((M, B, NN), dtype=c64, device=dev) if n > 128 else None
 
         # whole-sweep kernels (one launch per sweep) instead of one launch per slice
         self.fused = bool(fused)
@@ -166,6 +169,7 @@ class GroupWorkspace:
         d.grad_obj = None if self.grad_obj is None else self.grad_obj.data_ptr()
         d.grad_probe = None if self.grad_probe is None else self.grad_probe.data_ptr()
         d.chi_slices = None if self.chi_slices is None else self.chi_slices.data_ptr()
+        d.fft_scratch = None if self.fft_scratch is None else self.fft_scratch.data_ptr()
         return d
 
     # ------------------------------------------------------------------ host tables

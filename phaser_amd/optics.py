@@ -14,7 +14,7 @@ _E_REST = 5.1099906e5   # eV
 _HC = 1.23984244e4      # eV * angstrom
 
 # plane sides the FFT kernels are built for (phaser_amd/csrc/phx_fft.h)
-FFT_SIZES = (16, 32, 64, 128)
+FFT_SIZES = (16, 32, 64, 128, 192)
 
 
 def electron_wavelength(energy_ev: float) -> float:
@@ -72,8 +72,8 @@ def make_propagators(sampling, wavelength, thicknesses, bwlim_frac=2 / 3, dtype=
 
 
 def fft_lanes(n: int) -> int:
-    """Lanes per row/column of the FFT kernels for an n-point axis (8 at n = 128)."""
-    return 8 if n >= 128 else 4
+    """Lanes per row/column of the FFT kernels for an n-point axis (8 at n = 128, else 4)."""
+    return 8 if n == 128 else 4
 
 
 def scr_to_freq(n: int) -> numpy.ndarray:

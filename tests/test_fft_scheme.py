@@ -95,7 +95,7 @@ def inverse(s, n):
     return numpy.stack([_axis_inv(t[r], n) for r in range(n)])
 
 
-@pytest.mark.parametrize('n', [16, 32, 64, 128])
+@pytest.mark.parametrize('n', [16, 32, 64, 128, 192])
 def test_quad_scheme_scrambled_forward_and_natural_inverse(n):
     rng = numpy.random.default_rng(n)
     a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
@@ -114,6 +114,30 @@ def test_scrambled_order_small_case_by_hand():
     # n = 128, 8 lanes, E = 16: register k = 2r + s of lane q holds X[2q + s + 16r]
     f = scr_to_freq(128)
     assert list(f[:8]) == [0, 2, 4, 6, 8, 10, 12, 14] and f[8] == 1 and f[16] == 16 and f[127] == 127
+
+
+def _dft48(x, inverse=False):
+    """The kernel's in-register DFT-48 (phx_fft.h dft48): 48 = 3 x 16, decimation in
+    time, DFT-16s of x[3a + b], twiddles W_48^(b k), then a DFT-3 per k written as
+    a + s, (a - s/2) -+ i (sqrt(3)/2) d."""
+    sg = 1.0 if inverse else -1.0
+    y = [numpy.fft.ifft(x[b::3]) * 16 if inverse else numpy.fft.fft(x[b::3]) for b in range(3)]
+    k = numpy.arange(16)
+    y = [y[0]] + [y[b] * numpy.exp(sg * 2j * numpy.pi * b * k / 48) for b in (1, 2)]
+    sm, d = y[1] + y[2], y[1] - y[2]
+    t = y[0] - 0.5 * sm
+    r = (-sg) * -1j * (numpy.sqrt(3) / 2) * d   # -i h d forward, +i h d inverse
+    return numpy.concatenate([y[0] + sm, t + r, t - r])
+
+
+def test_dft48_split_matches_fft():
+    rng = numpy.random.default_rng(48)
+    x = rng.normal(size=48) + 1j * rng.normal(size=48)
+    numpy.testing.assert_allclose(_dft48(x), numpy.fft.fft(x), atol=1e-12)
+    numpy.testing.assert_allclose(_dft48(x, inverse=True), numpy.fft.ifft(x) * 48, atol=1e-12)
+    # n = 192: 4 lanes of 48 points; register k of lane q holds X[k + 48 bitrev2(q)]
+    f = scr_to_freq(192)
+    assert list(f[:4]) == [0, 96, 48, 144] and f[4] == 1 and f[191] == 191
 
 
 def test_pointwise_between_transforms_equals_fft_convolution():

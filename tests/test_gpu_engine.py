@@ -131,6 +131,15 @@ def test_n128_multislice_modes(solver):
     _check(*_run_all(state, pats, solver=solver, niter=2, grouping=8, beta=0.5))
 
 
+def test_n192_multislice_lsqml():
+    """192^2 planes (config 5's probe size: DFT-48 per lane, transposes through a global
+    scratch plane): 2 slices, 2 modes, 2 iterations of the fused LSQ-ML sweeps, against
+    the oracle."""
+    _, state, pats = _problem(n=192, n_modes=2, n_slices=2, scan_shape=(3, 3), scan_step_px=8.0, jitter_px=0.5,
+                              seed=12)
+    _check(*_run_all(state, pats, solver='lsqml', niter=2, grouping=9, beta=0.5))
+
+
 def test_lsqml_anscombe_multislice():
     _, state, pats = _problem(n=32, n_modes=2, n_slices=2, scan_shape=(8, 8), scan_step_px=3.0, jitter_px=0.5,
                               seed=6, defocus=30.0)

@@ -75,6 +75,24 @@ def test_adjoint_n128_matches_oracle_gradient():
     assert e[0] < 1e-5 and e[1] < 1e-4 and e[2] < 1e-4 and e[3] < 1e-3
 
 
+def test_adjoint_n192_matches_oracle_gradient():
+    """The fused adjoint and the gathered object gradient on 192^2 planes."""
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import NOISE_MODELS
+    _cfg, state, pats = _problem(S=2, M=2, n=192, scan=(2, 2))
+    group = numpy.array(numpy.unravel_index(numpy.arange(4), (2, 2)))
+    model, _ = make_model(state, pats.patterns, pats.pattern_mask, NOISE_MODELS['amplitude'](), groups=[group])
+    model.forward(0)
+    model.backward(0)
+    loss = float(model.loss_tensor(0).item())
+    o_loss, o_obj, o_probe, o_pos = _oracle_grad(state, pats, group, 'amplitude')
+    e = (abs(loss - o_loss) / abs(o_loss), rel_err(model.ws.grad_obj.cpu().numpy(), o_obj),
+         rel_err(model.ws.grad_probe.view(model.ws.probe.shape).cpu().numpy(), o_probe),
+         rel_err(model.position_grads(0)[:4].cpu().numpy(), o_pos))
+    print(f"n=192: loss {e[0]:.2e} object {e[1]:.2e} probe {e[2]:.2e} positions {e[3]:.2e}", flush=True)
+    assert e[0] < 1e-5 and e[1] < 1e-4 and e[2] < 1e-4 and e[3] < 1e-3
+
+
 def test_recompute_equals_store():
     from phaser_amd.engines.gradient import make_model
     from phaser_amd.plan import AmplitudeNoisePlan

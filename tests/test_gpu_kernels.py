@@ -22,7 +22,7 @@ def _st():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-@pytest.mark.parametrize('n', [16, 32, 64, 128])
+@pytest.mark.parametrize('n', [16, 32, 64, 128, 192])
 def test_fft2_ifft2_match_oracle(n):
     import torch
     from phaser_amd import _lib
@@ -163,7 +163,7 @@ def test_scramble_roundtrip():
     from phaser_amd import _lib
     from phaser_amd.optics import scramble
     rng = numpy.random.default_rng(3)
-    for n in (16, 32, 64, 128):
+    for n in (16, 32, 64, 128, 192):
         a = rng.normal(size=(2, n, n)).astype(numpy.float32)
         da = torch.from_numpy(a).cuda()
         out = torch.empty_like(da)
