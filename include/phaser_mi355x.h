@@ -114,7 +114,9 @@ typedef struct phx_group_desc {
   int32_t* finite_flag;      /* device int: min group index with NaN/inf (INT32_MAX = none) */
   int32_t psi_slots;         /* planes of psi kept per slot: >= S (or <= 0) stores every slice;
                                 3 keeps psi_0 plus two ping-pong slices (recompute mode) */
-  int32_t pad_;
+  int32_t prop_zero_waves;   /* bit w: every column that wave w of a plane workgroup owns in the
+                                column phase is zero in every propagator (band limit), so
+                                its column passes of a propagation are skipped; 0 = none */
   void* grad_obj;            /* (S, H, W) complex64 object gradient accumulator (gradient engine) */
   void* grad_probe;          /* (M, n, n) complex64 probe gradient (gradient engine) */
   void* chi_slices;          /* (S-1, bmax, M, n, n) complex64 or NULL.  When set, the

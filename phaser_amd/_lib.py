@@ -30,7 +30,7 @@ class GroupDesc(ctypes.Structure):
         ("psi", c_vp), ("chi_a", c_vp), ("chi_b", c_vp), ("farfield", c_vp), ("intensity", c_vp),
         ("gpm", c_vp), ("d_obj", c_vp), ("d_probe", c_vp), ("probe_k", c_vp), ("gom", c_vp),
         ("red", c_vp), ("errors", c_vp), ("aux", c_vp), ("finite_flag", c_vp),
-        ("psi_slots", c_i32), ("pad_", c_i32), ("grad_obj", c_vp), ("grad_probe", c_vp),
+        ("psi_slots", c_i32), ("prop_zero_waves", c_i32), ("grad_obj", c_vp), ("grad_probe", c_vp),
         ("chi_slices", c_vp),
         ("fft_scratch", c_vp),
     ]

@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int propag
   if (!propagate) return;
   const float2* prop = (const float2*)d.props;
   float2* psi1 = (float2*)d.psi + psi_off(d, 1, b, m);
-  pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); });
+  pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); }, d.prop_zero_waves);
   pl.store_rows([&](int yy, int xx, float2 v) { psi1[yy * N + xx] = cscale(v, inv_nn); }, x);
 }
 
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
     else step(std::false_type{});
     if (i == S - 1) break;
     const float2* prop = (const float2*)d.props + (size_t)i * NN;
-    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); });
+    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); }, d.prop_zero_waves);
   }
   pl.forward_regs(x);
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
@@ -583,7 +583,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   for (int i = S - 1; i > 0; --i) {
     pl.refresh();
     slice_step(i, std::false_type{}, std::false_type{});
-    pl.template convolve_prefetched<true>(x, (const float2*)d.props + (size_t)(i - 1) * NN);
+    pl.template convolve_prefetched<true>(x, (const float2*)d.props + (size_t)(i - 1) * NN, d.prop_zero_waves);
   }
   pl.refresh();
   slice_step(0, std::integral_constant<bool, PROBE>{}, std::false_type{});
@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     else step(std::false_type{});
     if (i == 0) break;
     const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
-    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); });
+    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); }, d.prop_zero_waves);
   }
   if (!WANT_P0) return;
   // slice 0: probe-gradient spectrum and position gradients (as gradient_backward_k)

@@ -463,31 +463,65 @@ struct Plane {
     inverse(src, dst, [] {});
   }
   // in-register forward -> pointwise mul(py, px, v) -> inverse
-  template <class Mul>
-  __device__ __forceinline__ void convolve_regs(float2 (&x)[E], Mul mul) {
-    forward_regs(x);
-    __builtin_amdgcn_sched_barrier(0);
+  // Column-phase waves whose columns are all zero in a convolution's multiplier (bit w of
+  // zero_waves for wave w: the host derives it from the band-limited propagators) skip
+  // the forward column pass, the product and the inverse column pass: their product is
+  // zero, and so is its inverse column transform (exactly, for finite data).  The test
+  // is wave-uniform (readfirstlane), and a skipping wave takes the one barrier the
+  // forward column pass would.
+  __device__ __forceinline__ bool wave_is_zero(unsigned zero_waves) const {
+    return (zero_waves >> __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) & 1u;
+  }
+  __device__ __forceinline__ void zero_column_pass(float2 (&x)[E]) {
+    if (QL == 8) __syncthreads();
 #pragma unroll
-    for (int k = 0; k < E; ++k) x[k] = mul(QL * k + q, rc, x[k]);
-    // the multiplier's loads (a propagator: L2-resident) all issued before the products
-    __builtin_amdgcn_sched_group_barrier(0x0020, 2 * E, 0);
-    __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    inverse_regs(x);
+    for (int k = 0; k < E; ++k) x[k] = make_float2(0.f, 0.f);
+  }
+  // in-register forward -> pointwise mul(py, px, v) -> inverse
+  template <class Mul>
+  __device__ __forceinline__ void convolve_regs(float2 (&x)[E], Mul mul, unsigned zero_waves = 0) {
+    fwd<false>(x);
+    rows_to_cols(x);
+    if (wave_is_zero(zero_waves)) {
+      zero_column_pass(x);
+    } else {
+      fwd<true>(x);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < E; ++k) x[k] = mul(QL * k + q, rc, x[k]);
+      // the multiplier's loads (a propagator: L2-resident) all issued before the products
+      __builtin_amdgcn_sched_group_barrier(0x0020, 2 * E, 0);
+      __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      inv<true>(x);
+    }
+    cols_to_rows(x);
+    inv<false>(x);
   }
   // in-register forward -> product with the scrambled-order plane mul (conjugated with
   // CONJ) -> inverse, the multiplier's loads issued before the forward transform so
   // their latency hides behind it
   template <bool CONJ>
-  __device__ __forceinline__ void convolve_prefetched(float2 (&x)[E], const float2* mul) {
+  __device__ __forceinline__ void convolve_prefetched(float2 (&x)[E], const float2* mul, unsigned zero_waves = 0) {
+    const bool zero = wave_is_zero(zero_waves);
     float2 mv[E];
+    if (!zero) {
 #pragma unroll
-    for (int k = 0; k < E; ++k) mv[k] = mul[(QL * k + q) * N + rc];
-    forward_regs(x);
-    __builtin_amdgcn_sched_barrier(0);
+      for (int k = 0; k < E; ++k) mv[k] = mul[(QL * k + q) * N + rc];
+    }
+    fwd<false>(x);
+    rows_to_cols(x);
+    if (zero) {
+      zero_column_pass(x);
+    } else {
+      fwd<true>(x);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < E; ++k) x[k] = CONJ ? cmulc(x[k], mv[k]) : cmul(x[k], mv[k]);
-    inverse_regs(x);
+      for (int k = 0; k < E; ++k) x[k] = CONJ ? cmulc(x[k], mv[k]) : cmul(x[k], mv[k]);
+      inv<true>(x);
+    }
+    cols_to_rows(x);
+    inv<false>(x);
   }
   template <class Src, class Mul, class Dst, class Pin>
   __device__ __forceinline__ void convolve(Src src, Mul mul, Dst dst, Pin pin) {

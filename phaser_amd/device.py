@@ -19,7 +19,7 @@ import ctypes
 import numpy
 
 from . import _lib
-from .optics import scr_to_freq, twiddles
+from .optics import fft_lanes, scr_to_freq, twiddles
 
 INT32_MAX = 2**31 - 1
 
@@ -53,6 +53,22 @@ def to_device_scrambled(torch, arr, n, is_complex, device):
                   b1 - b0, n, int(is_complex), 0, st)
     del src
     return out
+
+
+def zero_column_waves(props, n):
+    """Bitmask of the plane workgroup's waves whose column-phase columns (scrambled
+    positions) are zero in every propagator: the kernels skip their column passes in a
+    propagation.  `props` is (S-1, n, n) in natural frequency order, or None."""
+    if props is None:
+        return 0
+    L = fft_lanes(n)
+    cols_per_wave = 64 // L
+    zero_col = numpy.all(numpy.asarray(props).reshape(-1, n, n) == 0, axis=(0, 1))[scr_to_freq(n)]
+    mask = 0
+    for w in range(n * L // 64):
+        if zero_col[w * cols_per_wave:(w + 1) * cols_per_wave].all():
+            mask |= 1 << w
+    return mask
 
 
 class GroupWorkspace:
@@ -90,6 +106,7 @@ class GroupWorkspace:
         self.patterns_s = to_device_scrambled(torch, pats.astype(numpy.float32), n, False, dev)
         self.mask_s = to_device_scrambled(torch, numpy.asarray(pattern_mask, dtype=numpy.float32), n, False, dev).reshape(-1)
         self.props_s = None if props is None else to_device_scrambled(torch, props.astype(numpy.complex64), n, True, dev)
+        self.prop_zero_waves = zero_column_waves(props, n)
         # [beta_obj, beta_probe, reg_obj, reg_probe, gamma, offset, eps, var, loss_scale]
         self.params = torch.zeros(16, dtype=f32, device=dev)
         self.params_host = numpy.zeros(16, dtype=numpy.float32)
@@ -166,6 +183,7 @@ class GroupWorkspace:
         d.aux = self.aux.data_ptr()
         d.finite_flag = self.finite_flag.data_ptr()
         d.psi_slots = self.psi_slots
+        d.prop_zero_waves = self.prop_zero_waves
         d.grad_obj = None if self.grad_obj is None else self.grad_obj.data_ptr()
         d.grad_probe = None if self.grad_probe is None else self.grad_probe.data_ptr()
         d.chi_slices = None if self.chi_slices is None else self.chi_slices.data_ptr()
