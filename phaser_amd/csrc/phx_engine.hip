@@ -91,7 +91,7 @@ __device__ __forceinline__ float2* plane_scratch(const phx_group_desc& d) {
 #define PHX_PLANE_SETUP(N, T)                                                    \
   static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");               \
   __shared__ float2 lds_[Plane<N>::LDS_FLOAT2 + N];                              \
-  __shared__ double red_lds_[T / 64];                                            \
+  [[maybe_unused]] __shared__ double red_lds_[T / 64];                           \
   Plane<N> pl(Plane<N>::IN_LDS ? lds_ : plane_scratch<N>(d), lds_ + Plane<N>::LDS_FLOAT2); \
   load_twiddles<N>(lds_ + Plane<N>::LDS_FLOAT2, (const float2*)d.twiddle);       \
   __syncthreads();
@@ -192,7 +192,6 @@ __global__ void __launch_bounds__(T) forward_slice_k(phx_group_desc d, int i) {
   PHX_PLANE_SETUP(N, T);
   constexpr int NN = N * N;
   constexpr float inv_nn = 1.0f / (float)NN;
-  const int M = d.n_modes, bmax = d.bmax;
   const Cut c = cut_of(d, d.group_pos[b]);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
@@ -219,7 +218,7 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d) {
   PHX_PLANE_SETUP(N, T);
   constexpr int NN = N * N;
   constexpr float inv_nn = 1.0f / (float)NN;
-  const int M = d.n_modes, bmax = d.bmax, s = d.n_slices - 1;
+  const int M = d.n_modes, s = d.n_slices - 1;
   const Cut c = cut_of(d, d.group_pos[b]);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float2* psi_s = (const float2*)d.psi + psi_off(d, s, b, m);
