@@ -134,7 +134,9 @@ class GroupWorkspace:
         self.d_obj = torch.zeros((H, W), dtype=c64, device=dev)
         self.d_probe = torch.zeros((M, NN), dtype=c64, device=dev)
         self.probe_k = torch.empty((M, NN), dtype=c64, device=dev)
-        self.gom = torch.empty(NN, dtype=f32, device=dev)
+        # two object-magnitude planes, alternating by group slot: the clear of group g+1's
+        # plane follows group g's backward sweep while group g's probe step still reads its own
+        self.gom = torch.empty((2, NN), dtype=f32, device=dev)
         nred = int(_lib.load().phx_red_doubles(S, B, M))
         self.red = torch.zeros((self.G, nred), dtype=torch.float64, device=dev)
         self.errors = torch.zeros((self.G, B), dtype=f32, device=dev)
@@ -154,6 +156,8 @@ class GroupWorkspace:
         # a second stream (default priority) for the probe-side kernels of a group step
         self.side_stream = torch.cuda.Stream(device=dev)
         self.concurrent = True
+        # group slot whose accumulators the previous launch_lsqml_group call cleared
+        self._begun = None
         self.object = obj
         self.probe = probe
         self.descs = [self._make_desc(g) for g in range(self.G)]
@@ -177,7 +181,7 @@ class GroupWorkspace:
         d.psi, d.chi_a, d.chi_b = self.psi.data_ptr(), self.chi_a.data_ptr(), self.chi_b.data_ptr()
         d.farfield, d.intensity, d.gpm = self.farfield.data_ptr(), self.intensity.data_ptr(), self.gpm.data_ptr()
         d.d_obj, d.d_probe, d.probe_k = self.d_obj.data_ptr(), self.d_probe.data_ptr(), self.probe_k.data_ptr()
-        d.gom = self.gom.data_ptr()
+        d.gom = self.gom[g % 2].data_ptr()
         d.red = self.red[g].data_ptr()
         d.errors = self.errors[g].data_ptr()
         d.aux = self.aux.data_ptr()
@@ -233,6 +237,7 @@ class GroupWorkspace:
     def launch_dry_run(self, g):
         d, st = self.desc_refs[g], stream_handle()
         c = self.call
+        self._begun = None
         c("phx_group_begin", d, st)
         c("phx_probe_spectra", d, st)
         c("phx_shift_probes", d, 0, int(self.S > 1), st)
@@ -245,6 +250,7 @@ class GroupWorkspace:
         `modulus`) the far-field modulus update into chi_a."""
         d, st = self.desc_refs[g], stream_handle()
         c = self.call
+        self._begun = None
         c("phx_group_begin", d, st)
         c("phx_probe_spectra", d, st)
         if self.fused and not illum and self.psi_slots >= self.S:
@@ -269,26 +275,45 @@ class GroupWorkspace:
             # the sweeps and the object update: group_obj_mag needs only the zeroed
             # accumulators and the pre-update object, the probe step only the sweep's
             # probe spectra and alpha_P sums; both streams join before the next group.
+            #
+            # Capture order decides the hardware queue of every graph node: a node's first
+            # captured successor runs on its queue, later ones on other queues.  The order
+            # below keeps the chain begin -> forward -> backward -> update -> next forward
+            # on one queue (no cross-queue hop between the update and the next forward) and
+            # the probe-side kernels on a second one.  Group g+1's accumulators (intensity,
+            # its own reduction slot and error row, the object-magnitude plane of its
+            # parity) are free once group g's backward sweep has run, so they are cleared
+            # there; the streams join before each forward sweep, which needs the probe
+            # spectra of the previous group's probe step, and after the last group.
             side = self.side_stream if self.concurrent else None
             main = self.torch.cuda.current_stream()
             sh = ctypes.c_void_p(side.cuda_stream) if side is not None else st
-            c("phx_group_begin", d, st)
-            if update_probe:
-                if side is not None:
+            if g == 0 or self._begun != g:
+                c("phx_group_begin", d, st)
+            if side is not None:
+                main.wait_stream(side)
+                if update_probe:
+                    # group_obj_mag needs the cleared accumulator and the object after the
+                    # previous group's update, nothing of this group's forward sweep
                     side.wait_stream(main)
-                c("phx_group_obj_mag", d, sh)
             if g == 0:
                 # later groups find probe_k current: the previous group's probe step
                 # writes fft2 of the probe it updates, and without a probe update the
                 # probe is the one group 0 transformed
                 c("phx_probe_spectra", d, st)
             c("phx_group_forward", d, st)
+            if update_probe:
+                c("phx_group_obj_mag", d, sh)
             c("phx_lsqml_backward_sweep", d, int(update_object), int(update_probe), int(calc_error), st)
             if update_position:
                 self._launch_position(d, st)
+            if update_probe and side is not None:
+                side.wait_stream(main)
+            self._begun = None
+            if g + 1 < self.G:
+                c("phx_group_begin", self.desc_refs[g + 1], st)
+                self._begun = g + 1
             if update_probe:
-                if side is not None:
-                    side.wait_stream(main)
                 c("phx_probe_spectrum_reduce", d, sh)
                 if side is not None and update_object:
                     # the object update starts after the reduction, so the probe step (four
@@ -300,7 +325,7 @@ class GroupWorkspace:
                 # after the position step, which reads the pre-update slice-0 cutout
                 # (solvers.py:275-288)
                 c("phx_lsqml_object_update_slices", d, 0, self.S, st)
-            if side is not None:
+            if side is not None and g == self.G - 1:
                 main.wait_stream(side)
             return
         # illumination sums are gathered by the object update, not scattered here
