@@ -38,6 +38,12 @@ __device__ __forceinline__ void atomic_add_c64(float2* p, float2 v) {
   unsafeAtomicAdd(q + 1, v.y);
 }
 
+// Sum of a double over the 64 lanes of a wave; every lane gets the result.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // Block-wide sum of a double; every thread gets the result.  `scratch` holds
 // blockDim.x/64 doubles of LDS.  All threads of the block must call it.
 __device__ __forceinline__ double block_sum_f64(double v, double* scratch) {

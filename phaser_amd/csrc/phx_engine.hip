@@ -486,6 +486,9 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
 // or, at slice 0 with a probe update, the slot's probe spectrum fft2(dP) conj(filt_b)
 // into the farfield buffer with the alpha_P sums.  Object and probe updates follow in
 // their own launches; the sweep reads only the pre-update cutouts (solvers.py:246-260).
+// slices whose alpha partials the fused backward sweep keeps per wave in LDS
+constexpr int BK_WRED_SLICES = 32;
+
 template <int N, int T, bool UPD_OBJ, bool PROBE>
 __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, int calc_error) {
   const int m = blockIdx.x, b = blockIdx.y;
@@ -499,6 +502,37 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   const Cut c = cut_of(d, p);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const RedLayout L = red_layout(S, bmax, M);
+  // alpha sums: each wave leaves its FP64 partials of a slice step in its own LDS slots
+  // (slice i at slot i, the probe sums at slot S) and the slots are summed once, after
+  // the sweep, so the slice steps take no block-wide barrier for them.  Deeper stacks
+  // than the slots hold sum every step across the block instead.
+  constexpr int NW = T / 64;
+  __shared__ double wred_lds_[(BK_WRED_SLICES + 1) * 2 * NW];
+  const bool deferred = S <= BK_WRED_SLICES;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  auto stash = [&](int slot, float v0, float v1, auto flush) __attribute__((always_inline)) {
+    if (deferred) {
+      const double a = wave_sum_f64(v0), bb = wave_sum_f64(v1);
+      if (lane == 0) {
+        wred_lds_[(2 * slot) * NW + wave] = a;
+        wred_lds_[(2 * slot + 1) * NW + wave] = bb;
+      }
+    } else {
+      const double a = block_sum_f64(v0, red_lds_);
+      const double bb = block_sum_f64(v1, red_lds_);
+      if (threadIdx.x == 0) flush(a, bb);
+    }
+  };
+  auto flush_obj = [&](int i) {
+    return [&, i](double a, double bb) {
+      atomic_add_f64(&d.red[L.num_obj + i * bmax + b], a);
+      atomic_add_f64(&d.red[L.den_obj + i], bb);
+    };
+  };
+  auto flush_probe = [&](double a, double bb) {
+    d.red[L.numP + b * M + m] = a;
+    atomic_add_f64(&d.red[L.denP], bb);
+  };
 
   float2 x[E];
   // far field: chi_hat = (sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask F
@@ -560,22 +594,8 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
       if (UPD_OBJ) __asm__ volatile("" : "+v"(numO), "+v"(denO));
       if (PS) __asm__ volatile("" : "+v"(numP), "+v"(denP));
     });
-    if (UPD_OBJ) {
-      const double a = block_sum_f64(numO, red_lds_);
-      const double bb = block_sum_f64(denO, red_lds_);
-      if (threadIdx.x == 0) {
-        atomic_add_f64(&d.red[L.num_obj + i * bmax + b], a);
-        atomic_add_f64(&d.red[L.den_obj + i], bb);
-      }
-    }
-    if (PS) {
-      const double a = block_sum_f64(numP, red_lds_);
-      const double bb = block_sum_f64(denP, red_lds_);
-      if (threadIdx.x == 0) {
-        d.red[L.numP + b * M + m] = a;
-        atomic_add_f64(&d.red[L.denP], bb);
-      }
-    }
+    if (UPD_OBJ) stash(i, numO, denO, flush_obj(i));
+    if (PS) stash(S, numP, denP, flush_probe);
   };
   // (cutouts read through obj_at: an interior-only copy of the sweep would need more
   // registers than the plane allows alongside the general one)
@@ -598,6 +618,21 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
       const int py = pl.QL * k + pl.q;
       Fp[py * N + px] = cmulc(x[k], shift_filter(sy, sx, d.ky_s[py], kx));
       if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // a few phase factors at a time
+    }
+  }
+  if (deferred && (UPD_OBJ || PROBE)) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * (S + 1); t += T) {
+      const int slot = t >> 1;
+      if (slot < S ? !UPD_OBJ : !PROBE) continue;
+      double sum = 0.0;
+      for (int w = 0; w < NW; ++w) sum += wred_lds_[t * NW + w];
+      if (slot == S) {
+        if (t & 1) atomic_add_f64(&d.red[L.denP], sum);
+        else d.red[L.numP + b * M + m] = sum;
+      } else {
+        atomic_add_f64(t & 1 ? &d.red[L.den_obj + slot] : &d.red[L.num_obj + slot * bmax + b], sum);
+      }
     }
   }
 }

@@ -166,6 +166,16 @@ def test_band_limit_column_skip_matches(monkeypatch):
     assert rel(a.probe.data, b.probe.data) < 1e-5
 
 
+def test_lsqml_deep_stack_block_sums():
+    """34 slices: more than the fused backward sweep keeps per-wave alpha partials for in
+    LDS (32), so every slice step sums its alpha terms across the block instead; 2
+    iterations against the oracle.  A stack this deep is ill-conditioned in FP32 (the
+    FP32 oracle's probe differs from FP64 by ~2e-3), so the FP64-relative bar."""
+    _, state, pats = _problem(n=32, n_modes=1, n_slices=34, scan_shape=(6, 6), scan_step_px=4.0, jitter_px=0.5,
+                              seed=21)
+    _check(*_run_all(state, pats, solver='lsqml', niter=2, grouping=36, beta=0.5), strict=False)
+
+
 def test_n192_multislice_lsqml():
     """192^2 planes (config 5's probe size: DFT-48 per lane, transposes through a global
     scratch plane): 2 slices, 2 modes, 2 iterations of the fused LSQ-ML sweeps, against
