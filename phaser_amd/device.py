@@ -285,17 +285,17 @@ class GroupWorkspace:
             # parity) are free once group g's backward sweep has run, so they are cleared
             # there; the streams join before each forward sweep, which needs the probe
             # spectra of the previous group's probe step, and after the last group.
-            side = self.side_stream if self.concurrent else None
+            # (an iteration without a probe update runs on the caller's stream alone)
+            side = self.side_stream if self.concurrent and update_probe else None
             main = self.torch.cuda.current_stream()
             sh = ctypes.c_void_p(side.cuda_stream) if side is not None else st
             if g == 0 or self._begun != g:
                 c("phx_group_begin", d, st)
             if side is not None:
                 main.wait_stream(side)
-                if update_probe:
-                    # group_obj_mag needs the cleared accumulator and the object after the
-                    # previous group's update, nothing of this group's forward sweep
-                    side.wait_stream(main)
+                # group_obj_mag needs the cleared accumulator and the object after the
+                # previous group's update, nothing of this group's forward sweep
+                side.wait_stream(main)
             if g == 0:
                 # later groups find probe_k current: the previous group's probe step
                 # writes fft2 of the probe it updates, and without a probe update the
@@ -307,7 +307,7 @@ class GroupWorkspace:
             c("phx_lsqml_backward_sweep", d, int(update_object), int(update_probe), int(calc_error), st)
             if update_position:
                 self._launch_position(d, st)
-            if update_probe and side is not None:
+            if side is not None:
                 side.wait_stream(main)
             self._begun = None
             if g + 1 < self.G:

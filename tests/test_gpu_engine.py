@@ -122,6 +122,26 @@ def test_lsqml_multislice_modes_with_graphs():
     _check(*_run_all(state, pats, solver='lsqml', niter=5, grouping=48, use_graphs=True))
 
 
+def test_lsqml_graphs_probe_update_switching_on():
+    """Graphs captured both without the probe update (iterations 0-1: the group steps run
+    on the caller's stream alone) and with it (iterations 2-3: probe-side kernels on the
+    second stream), replayed in turn, against the oracle with the same flag."""
+    from phaser_amd.engines.conventional import run_engine
+    from phaser_amd.plan import ConventionalEnginePlan, SOLVERS
+    _, state, pats = _problem(n=64, n_modes=2, n_slices=3, scan_shape=(8, 8), scan_step_px=6.0, jitter_px=0.5,
+                              seed=5)
+    flag = {'after': 1}
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver=SOLVERS['lsqml'](beta_object=0.5, beta_probe=0.5),
+                                  niter=4, grouping=16, update_probe=flag)
+    g = run_engine({'data': pats, 'state': state.copy(), 'dtype': numpy.float32, 'seed': 7, 'observer': None,
+                    'recons_name': 't'}, plan, use_graphs=True).to_numpy()
+    kw = dict(solver='lsqml', noise_model=onoise.AmplitudeNoiseModel(), niter=4, grouping=16, seed=7,
+              beta_object=0.5, beta_probe=0.5, update_probe=flag)
+    o32 = _run_oracle(state, pats, numpy.float32, **kw)
+    o64 = _run_oracle(state, pats, numpy.float64, **kw)
+    _check(g, o32, o64)
+
+
 @pytest.mark.parametrize('solver', ['lsqml', 'epie'])
 def test_n128_multislice_modes(solver):
     """128^2 planes (the benchmark's size, whose transforms use 8 lanes per row): 3 slices,
