@@ -202,6 +202,10 @@ int phx_probe_spectrum_reduce(const phx_group_desc* d, hipStream_t stream);
 /* P += beta_P (sum_b alpha_P) ifft2(d_probe)/(gom + reg) * gom/(gom + 1e-16); d_probe = 0;
  * then probe_k = fft2(P) of the updated probe (as phx_probe_spectra). */
 int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream);
+/* The same probe step (same result) in three launches of small workgroups (strips of
+ * lines of the plane) that run beside a concurrent object update instead of each
+ * needing a whole CU; tmp is an n_modes * n * n complex64 work plane.  n <= 128. */
+int phx_lsqml_probe_apply_strips(const phx_group_desc* d, void* tmp, hipStream_t stream);
 
 /* ePIE backward step at slice i (solvers.py:459-485), maxima first, then the update. */
 int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,

@@ -79,6 +79,7 @@ SIGNATURES = {
     "phx_lsqml_object_update_slices": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_probe_spectrum_reduce": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply": (c_int, [_P, c_vp]),
+    "phx_lsqml_probe_apply_strips": (c_int, [_P, c_vp, c_vp]),
     "phx_epie_backward": (c_int, [_P, c_int, c_int, c_int, c_vp]),
     "phx_epie_object_apply": (c_int, [_P, c_int, c_vp]),
     "phx_epie_probe_apply": (c_int, [_P, c_vp]),

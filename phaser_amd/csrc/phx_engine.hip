@@ -958,6 +958,116 @@ __global__ void __launch_bounds__(T) lsqml_probe_apply_k(phx_group_desc d) {
   for (int k = 0; k < E; ++k) Pk[(pl.QL * k + pl.q) * N + pl.rc] = x[k];
 }
 
+// The same probe step split over three launches of strips (R lines of a plane per
+// workgroup, QL lanes per line: 128 threads at n = 128), each with a footprint like one
+// object-update workgroup, so that they run beside the object update as its workgroups
+// retire instead of waiting for whole free CUs.  A 1-D transform along a lane group is the
+// plane's own fwd/inv (the in-wave exchange uses this wave's slots of a strip-sized LDS
+// buffer), and the layouts in tmp are those the plane's transposes produce, so every
+// value is computed as in lsqml_probe_apply_k:
+//   cols_inv:  tmp[y][px]   = inverse column transform of d_probe (scrambled py -> y)
+//   rows:      P += update(inverse row transform of tmp) ; tmp[y][px] = forward row
+//              transform of the new P (x -> scrambled px)
+//   cols_fwd:  probe_k      = forward column transform of tmp (y -> scrambled py)
+template <int N>
+struct Strip {
+  static constexpr int QL = Plane<N>::QL;
+  static constexpr int R = (128 / QL) < N ? (128 / QL) : N;  // lines per workgroup
+  static constexpr int T = R * QL;
+  static constexpr int LDS_FLOAT2 = QL == 8 ? R * Plane<N>::LD : 0;  // exchange slots
+};
+
+#define PHX_STRIP_SETUP(N)                                                                   \
+  using SG = Strip<N>;                                                                       \
+  __shared__ float2 slds_[SG::LDS_FLOAT2 + N];                                               \
+  Plane<N> pl(slds_, slds_ + SG::LDS_FLOAT2);                                                \
+  for (int i = threadIdx.x; i < N; i += SG::T)                                               \
+    slds_[SG::LDS_FLOAT2 + i] = ((const float2*)d.twiddle)[(i % SG::QL) * (i / SG::QL)];     \
+  __syncthreads();                                                                           \
+  const int line = blockIdx.x * SG::R + pl.rc;                                               \
+  const int m = blockIdx.y;                                                                  \
+  constexpr int NN = N * N;                                                                  \
+  constexpr int E = Plane<N>::E;                                                             \
+  constexpr int QL = SG::QL;
+
+template <int N>
+__global__ void __launch_bounds__(Strip<N>::T) probe_apply_cols_inv_k(phx_group_desc d, float2* tmp) {
+  PHX_STRIP_SETUP(N);
+  float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+  float2* t = tmp + (size_t)m * NN;
+  float2 x[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) x[k] = dPk[(QL * k + pl.q) * N + line];
+#pragma unroll
+  for (int k = 0; k < E; ++k) dPk[(QL * k + pl.q) * N + line] = make_float2(0.f, 0.f);
+  pl.template inv<false>(x);
+#pragma unroll
+  for (int j = 0; j < E; ++j) t[(QL * j + pl.q) * N + line] = x[j];
+}
+
+template <int N>
+__global__ void __launch_bounds__(Strip<N>::T) probe_apply_rows_k(phx_group_desc d, float2* tmp) {
+  PHX_STRIP_SETUP(N);
+  constexpr float inv_nn = 1.0f / (float)NN;
+  __shared__ float sA;
+  const int M = d.n_modes;
+  if (threadIdx.x < 64) {
+    const RedLayout L = red_layout(d.n_slices, d.bmax, M);
+    const int gs = *d.group_size;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < gs; b += 64) s += d.red[L.numP + b * M + m];
+    s = wave_sum_f64(s);
+    if (threadIdx.x == 0) {
+      const double den = d.red[L.denP] + (double)d.params[P_GAMMA];
+      sA = (float)((double)d.params[P_BETA_PROBE] * (s / den));
+    }
+  }
+  float2* t = tmp + (size_t)m * NN;
+  float2* P = (float2*)d.probe + (size_t)m * NN;
+  float2 x[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) x[k] = t[line * N + QL * k + pl.q];
+  __syncthreads();  // sA
+  const float A = sA, reg = d.params[P_REG_PROBE];
+  pl.template inv<false>(x);
+  float2 pv[E];
+  float g[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int e = line * N + QL * j + pl.q;
+    g[j] = d.gom[e];
+    pv[j] = P[e];
+  }
+  __builtin_amdgcn_sched_barrier(0);  // every load above issued before the first store
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const float2 avg = cscale(cscale(x[j], inv_nn), 1.0f / (g[j] + reg));
+    const float2 u = cscale(cscale(cscale(avg, A), g[j]), 1.0f / (g[j] + 1e-16f));
+    const float2 o = cadd(pv[j], u);
+    P[line * N + QL * j + pl.q] = o;
+    bad |= !(isfinite(o.x) && isfinite(o.y));
+    x[j] = o;
+  }
+  if (bad) atomicMin(d.finite_flag, d.group_index);
+  pl.template fwd<false>(x);
+#pragma unroll
+  for (int k = 0; k < E; ++k) t[line * N + QL * k + pl.q] = x[k];
+}
+
+template <int N>
+__global__ void __launch_bounds__(Strip<N>::T) probe_apply_cols_fwd_k(phx_group_desc d, float2* tmp) {
+  PHX_STRIP_SETUP(N);
+  const float2* t = tmp + (size_t)m * NN;
+  float2* Pk = (float2*)d.probe_k + (size_t)m * NN;
+  float2 x[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) x[j] = t[(QL * j + pl.q) * N + line];
+  pl.template fwd<false>(x);
+#pragma unroll
+  for (int k = 0; k < E; ++k) Pk[(QL * k + pl.q) * N + line] = x[k];
+}
+
 // ---------------------------------- ePIE ------------------------------------------
 // aux[b] = max_px |O_i|^2, aux[bmax + b] = max_px sum_m |psi_i|^2 (solvers.py:462-472)
 __global__ void __launch_bounds__(256) epie_maxima_k(phx_group_desc d, int i) {
@@ -1572,6 +1682,32 @@ int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream) {
   if (!d->d_probe || !d->gom || !d->finite_flag) return PHX_EINVAL;
 #define L(N, T) hipLaunchKernelGGL((lsqml_probe_apply_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_probe_apply_strips(const phx_group_desc* d, void* tmp, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->d_probe || !d->gom || !d->finite_flag || !d->probe_k || !tmp) return PHX_EINVAL;
+  float2* t = (float2*)tmp;
+#define L(N, T_)                                                                                         \
+  {                                                                                                      \
+    const dim3 grid((unsigned)(N / Strip<N>::R), (unsigned)d->n_modes), block(Strip<N>::T);              \
+    hipLaunchKernelGGL((probe_apply_cols_inv_k<N>), grid, block, 0, stream, *d, t);                      \
+    PHX_CHECK_LAUNCH();                                                                                  \
+    hipLaunchKernelGGL((probe_apply_rows_k<N>), grid, block, 0, stream, *d, t);                          \
+    PHX_CHECK_LAUNCH();                                                                                  \
+    hipLaunchKernelGGL((probe_apply_cols_fwd_k<N>), grid, block, 0, stream, *d, t);                      \
+  }
+  switch (d->n) {
+    case 16: L(16, 0); break;
+    case 32: L(32, 0); break;
+    case 64: L(64, 0); break;
+    case 128: L(128, 0); break;
+    default: return PHX_EUNSUPPORTED;
+  }
 #undef L
   PHX_CHECK_LAUNCH();
   return PHX_OK;

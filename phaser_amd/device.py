@@ -22,6 +22,8 @@ from . import _lib
 from .optics import fft_lanes, scr_to_freq, twiddles
 
 INT32_MAX = 2**31 - 1
+# largest plane whose LSQ-ML probe step runs in strips (phx_lsqml_probe_apply_strips)
+PROBE_STRIPS_MAX_N = 128
 
 
 def _require_torch_gpu():
@@ -150,6 +152,9 @@ class GroupWorkspace:
                            if keep_chi and S > 1 else None)
         # planes too large for LDS are transposed through one scratch plane per workgroup
         self.fft_scratch = torch.empty((M, B, NN), dtype=c64, device=dev) if n > 128 else None
+        # the work plane of the probe step in strips (n <= PROBE_STRIPS_MAX_N)
+        self.probe_strips = n <= PROBE_STRIPS_MAX_N
+        self.probe_tmp = torch.empty((M, NN), dtype=c64, device=dev) if self.probe_strips else None
 
         # whole-sweep kernels (one launch per sweep) instead of one launch per slice
         self.fused = bool(fused)
@@ -313,18 +318,27 @@ class GroupWorkspace:
             if g + 1 < self.G:
                 c("phx_group_begin", self.desc_refs[g + 1], st)
                 self._begun = g + 1
-            if update_probe:
-                c("phx_probe_spectrum_reduce", d, sh)
-                if side is not None and update_object:
-                    # the object update starts after the reduction, so the probe step (four
-                    # workgroups that each need a whole CU) is dispatched before the update
-                    # fills every CU
-                    main.wait_stream(side)
-                c("phx_lsqml_probe_apply", d, sh)
-            if update_object:
-                # after the position step, which reads the pre-update slice-0 cutout
-                # (solvers.py:275-288)
-                c("phx_lsqml_object_update_slices", d, 0, self.S, st)
+            if self.probe_strips:
+                # the probe step in strips of small workgroups, which run beside the object
+                # update as its workgroups retire: the update starts right away
+                if update_object:
+                    # after the position step, which reads the pre-update slice-0 cutout
+                    # (solvers.py:275-288)
+                    c("phx_lsqml_object_update_slices", d, 0, self.S, st)
+                if update_probe:
+                    c("phx_probe_spectrum_reduce", d, sh)
+                    c("phx_lsqml_probe_apply_strips", d, _ptr(self.probe_tmp), sh)
+            else:
+                if update_probe:
+                    c("phx_probe_spectrum_reduce", d, sh)
+                    if side is not None and update_object:
+                        # the object update starts after the reduction, so the probe step
+                        # (four workgroups that each need a whole CU) is dispatched before
+                        # the update fills every CU
+                        main.wait_stream(side)
+                    c("phx_lsqml_probe_apply", d, sh)
+                if update_object:
+                    c("phx_lsqml_object_update_slices", d, 0, self.S, st)
             if side is not None and g == self.G - 1:
                 main.wait_stream(side)
             return

@@ -196,6 +196,35 @@ def test_lsqml_deep_stack_block_sums():
     _check(*_run_all(state, pats, solver='lsqml', niter=2, grouping=36, beta=0.5), strict=False)
 
 
+def test_probe_step_strips_match_single_launch(monkeypatch):
+    """The probe step in three strip launches (beside the object update) against the
+    one-workgroup-per-mode launch after the reduction, at 128^2 and 64^2 with graphs:
+    equal to the run-to-run noise of the path (atomic sums)."""
+    import phaser_amd.device as dev
+    from phaser_amd.engines.conventional import run_engine
+    from phaser_amd.plan import ConventionalEnginePlan, SOLVERS
+    for n, scan, grouping in ((128, (4, 4), 8), (64, (8, 8), 16)):
+        _, state, pats = _problem(n=n, n_modes=2, n_slices=3, scan_shape=scan, scan_step_px=6.0, jitter_px=0.5,
+                                  seed=8)
+        plan = ConventionalEnginePlan(noise_model='amplitude',
+                                      solver=SOLVERS['lsqml'](beta_object=0.5, beta_probe=0.5), niter=3,
+                                      grouping=grouping)
+        args = {'data': pats, 'dtype': numpy.float32, 'seed': 7, 'observer': None, 'recons_name': 't'}
+        monkeypatch.setattr(dev, 'PROBE_STRIPS_MAX_N', 128)
+        a = run_engine(dict(args, state=state.copy()), plan, use_graphs=True).to_numpy()
+        a2 = run_engine(dict(args, state=state.copy()), plan, use_graphs=True).to_numpy()
+        monkeypatch.setattr(dev, 'PROBE_STRIPS_MAX_N', 0)
+        b = run_engine(dict(args, state=state.copy()), plan, use_graphs=True).to_numpy()
+
+        def rel(x, y):
+            return float(numpy.abs(x - y).max() / numpy.abs(y).max())
+        noise = max(rel(a.object.data, a2.object.data), rel(a.probe.data, a2.probe.data))
+        print(n, 'noise', noise, 'strips vs single', rel(a.probe.data, b.probe.data), flush=True)
+        assert noise < 1e-5
+        assert rel(a.object.data, b.object.data) < 1e-5
+        assert rel(a.probe.data, b.probe.data) < 1e-5
+
+
 def test_n192_multislice_lsqml():
     """192^2 planes (config 5's probe size: DFT-48 per lane, transposes through a global
     scratch plane): 2 slices, 2 modes, 2 iterations of the fused LSQ-ML sweeps, against
