@@ -249,6 +249,12 @@ int phx_farfield_gradient_update(const phx_group_desc* d, int poisson, hipStream
  * stored (not the recompute mode) and, for the object gradient with S > 1, chi_slices. */
 int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe, int want_positions,
                        hipStream_t stream);
+/* phx_farfield_gradient_update and phx_gradient_sweep in one launch (the same values):
+ * the far-field gradient step and the group loss computed in the sweep's first column
+ * pass, the exit-wave gradient transformed in registers (and stored in chi_a with
+ * want_object).  For callers that need the loss only after the adjoint pass. */
+int phx_gradient_sweep_farfield(const phx_group_desc* d, int poisson, int want_object, int want_probe,
+                                int want_positions, hipStream_t stream);
 /* grad_obj[i] = sum_{b,m} scatter(g_i conj(psi_i)) for slices first .. first+count-1
  * from the wave gradients phx_gradient_sweep stored, gathered per object tile in group
  * and mode order (no atomics; deterministic).  Writes every pixel of those slices (zero
@@ -259,6 +265,10 @@ int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream);
 /* Gradient-engine rescale factors into errors[b]: sum(D) / sum |psi_{S-1} O_{S-1}|^2
  * (gradient/run.py:408-438: last slice included, pattern mask not applied). */
 int phx_dry_run_stats_gradient(const phx_group_desc* d, hipStream_t stream);
+/* The same ratio from the model intensity phx_group_forward left (sum over the plane of
+ * sum_m |F_m|^2 / n^2 = sum_m |psi_{S-1} O_{S-1}|^2 by Parseval): the dry run as one
+ * forward launch and this reduction. */
+int phx_dry_run_stats_intensity(const phx_group_desc* d, hipStream_t stream);
 
 /* Optimizer steps on complex64 parameters (gradient/solvers.py:88-138 over optax): the
  * direction is d = -grad / scale; finite_flag (may be NULL) records flag_value on NaN/inf.

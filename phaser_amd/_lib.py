@@ -64,6 +64,8 @@ SIGNATURES = {
     "phx_lsqml_object_update": (c_int, [_P, c_int, c_vp]),
     "phx_farfield_gradient_update": (c_int, [_P, c_int, c_vp]),
     "phx_gradient_sweep": (c_int, [_P, c_int, c_int, c_int, c_vp]),
+    "phx_dry_run_stats_intensity": (c_int, [_P, c_vp]),
+    "phx_gradient_sweep_farfield": (c_int, [_P, c_int, c_int, c_int, c_int, c_vp]),
     "phx_gradient_object_gather": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_gemm_bf16": (c_int, [c_vp, c_i64, c_int, c_vp, c_i64, c_int, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp,
                               c_int, c_vp, c_i64, c_int, c_vp]),

@@ -248,6 +248,41 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d) {
 //  FM_GRAD_AMPLITUDE / FM_GRAD_POISSON: gradient of the amplitude (noise_models.py:29-45)
 //    or Poisson (:86-102) loss w.r.t. the exit wave, in torch's convention
 //    (2 dL/dconj(a)), times params[P_LOSS_SCALE]; the group loss (FP64) goes to red.loss.
+// Per-point far-field factor u (the update multiplies F by it) and the point's error or
+// loss term t, for the modes above.
+template <int MODE>
+__device__ __forceinline__ float farfield_factor(float Iv, float Dv, float mk, float off, float eps, float var,
+                                                 float gscale, float& t) {
+  const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
+  float u;
+  if (MODE == FM_MODULUS) {
+    const float df = Iv - Dv;
+    t = df * df;  // unmasked (solvers.py:237), NaN-skipping (nansum)
+    t = isnan(t) ? 0.f : t;
+    u = (sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f) * mk;
+  } else if (MODE == FM_GRAD_AMPLITUDE) {
+    const float si = sqrtf(Iv + off);
+    const float sd = sqrtf(Dp + off) - si - eps;
+    t = mk * sd * sd;
+    // at exactly zero model intensity every Psi_m is 0 and |Psi|/sqrt(I) <= 1: the
+    // zero subgradient (autodiff of sqrt at 0 would give inf * 0 = NaN)
+    u = si > 0.f ? -4.0f * mk * sd / (var * si) * gscale : 0.f;
+  } else {
+    t = mk * (Iv + eps + Dp * (logf(Dp + eps) - logf(Iv + eps) - 1.0f));
+    u = 2.0f * mk * (1.0f - Dp / (Iv + eps)) * gscale;
+  }
+  return u;
+}
+
+// The group loss of the gradient modes from the block's FP32 partials (mode-0 planes
+// only): FP64 block sum, amplitude scaling, one atomic per plane into red.loss.
+template <int MODE>
+__device__ __forceinline__ void add_group_loss(const phx_group_desc& d, float e2, double* red_lds) {
+  double tot = block_sum_f64(e2, red_lds);
+  if (MODE == FM_GRAD_AMPLITUDE) tot *= 2.0 / (double)d.params[P_VAR];
+  if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(d.n_slices, d.bmax, d.n_modes).loss], tot);
+}
+
 template <int N, int T, int MODE>
 __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int calc_error) {
   const int m = blockIdx.x, b = blockIdx.y;
@@ -273,36 +308,19 @@ __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int cal
         const int e = py * N + px;
         const float Iv = I[e], Dv = D[e], mk = mask[e];
         const float2 Fv = F[e];
-        const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
-        float u, t;
-        if (MODE == FM_MODULUS) {
-          const float df = Iv - Dv;
-          t = df * df;  // unmasked (solvers.py:237), NaN-skipping (nansum)
-          t = isnan(t) ? 0.f : t;
-          u = (sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f) * mk;
-        } else if (MODE == FM_GRAD_AMPLITUDE) {
-          const float si = sqrtf(Iv + off);
-          const float sd = sqrtf(Dp + off) - si - eps;
-          t = mk * sd * sd;
-          // at exactly zero model intensity every Psi_m is 0 and |Psi|/sqrt(I) <= 1: the
-          // zero subgradient (autodiff of sqrt at 0 would give inf * 0 = NaN)
-          u = si > 0.f ? -4.0f * mk * sd / (var * si) * gscale : 0.f;
-        } else {
-          t = mk * (Iv + eps + Dp * (logf(Dp + eps) - logf(Iv + eps) - 1.0f));
-          u = 2.0f * mk * (1.0f - Dp / (Iv + eps)) * gscale;
-        }
+        float t;
+        const float u = farfield_factor<MODE>(Iv, Dv, mk, off, eps, var, gscale, t);
         e2 += t;
         return cscale(Fv, u);
       },
       [&](int y, int x, float2 v) { chi[y * N + x] = cscale(v, inv_nn); },
       [&] { __asm__ volatile("" : "+v"(e2)); });
   if (err) {
-    double tot = block_sum_f64(e2, red_lds_);
     if (MODE == FM_MODULUS) {
+      const double tot = block_sum_f64(e2, red_lds_);
       if (threadIdx.x == 0) d.errors[b] = (float)sqrt(tot);
     } else {
-      if (MODE == FM_GRAD_AMPLITUDE) tot *= 2.0 / (double)var;
-      if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(d.n_slices, d.bmax, M).loss], tot);
+      add_group_loss<MODE>(d, e2, red_lds_);
     }
   }
 }
@@ -644,7 +662,11 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
 // afterwards by gradient_object_gather_k, without atomics), g <- g conj(O_i), and
 // g <- ifft2(fft2(g) conj(p_{i-1})); at slice 0 the probe-gradient spectrum and the
 // position gradients.  Needs every psi_i stored and, with S > 1, chi_slices.
-template <int N, int T, bool WANT_OBJ, bool WANT_P0>
+// FF = 0 starts from chi_a as phx_farfield_gradient_update left it; FF = FM_GRAD_AMPLITUDE
+// or FM_GRAD_POISSON computes that far-field gradient step itself (the same values, the
+// group loss included) in the first column pass and transforms it in registers, and
+// stores slice S-1's wave gradient into chi_a.
+template <int N, int T, bool WANT_OBJ, bool WANT_P0, int FF>
 __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want_probe, int want_pos) {
   const int m = blockIdx.x, b = blockIdx.y;
   if (!slot_active(d, b)) return;
@@ -659,10 +681,29 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   const bool inside = c.interior(N);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   float2 x[E];
-  {
+  if constexpr (FF == 0) {
     // the stored exit-wave gradient, times n^2 (a power of two: exact) to carry it unscaled
     const float2* g0 = (const float2*)d.chi_a + ((size_t)b * M + m) * NN;
     pl.rows_apply(x, [&](int yy, int xx, float2) { return cscale(g0[yy * N + xx], (float)NN); }, [] {});
+  } else {
+    const float2* F = (const float2*)d.farfield + ((size_t)b * M + m) * NN;
+    const float* I = d.intensity + (size_t)b * NN;
+    const float* D = d.patterns_s + (size_t)p * NN;
+    const float* mask = d.mask_s;
+    const float off = d.params[P_OFFSET], eps = d.params[P_EPS], var = d.params[P_VAR];
+    const float gscale = d.params[P_LOSS_SCALE];
+    float e2 = 0.f;
+    pl.cols_apply(x, [&](int py, int px, float2) {
+      const int e = py * N + px;
+      const float Iv = I[e], Dv = D[e], mk = mask[e];
+      const float2 Fv = F[e];
+      float t;
+      const float u = farfield_factor<FF>(Iv, Dv, mk, off, eps, var, gscale, t);
+      e2 += t;
+      return cscale(Fv, u);
+    }, [&] { __asm__ volatile("" : "+v"(e2)); });
+    if (m == 0) add_group_loss<FF>(d, e2, red_lds_);
+    pl.inverse_regs(x);
   }
   for (int i = S - 1; i >= 0; --i) {
     pl.refresh();
@@ -671,7 +712,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     // the wave gradient at slice i goes to chi_i (chi_a itself at i = S-1), for
     // gradient_object_gather_k to form grad_O_i without atomics
     float2* gi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
-    const bool store = WANT_OBJ && i < S - 1;
+    const bool store = WANT_OBJ && (FF != 0 || i < S - 1);
     auto step = [&](auto interior_tag) __attribute__((always_inline)) {
       constexpr bool IN = decltype(interior_tag)::value;
       const float2* orow = obj_i + (size_t)(c.y0 + yy) * c.W + c.x0;  // interior cutouts
@@ -1339,6 +1380,25 @@ __global__ void __launch_bounds__(256) dry_run_stats_gradient_k(phx_group_desc d
   if (threadIdx.x == 0) d.errors[b] = (float)ex / (float)model;
 }
 
+// The same ratio after group_forward_k: the slot's model intensity sum_m |F_m|^2 / n^2
+// summed over the plane equals sum_m |psi_{S-1} O_{S-1}|^2 (Parseval).
+__global__ void __launch_bounds__(256) dry_run_stats_intensity_k(phx_group_desc d) {
+  __shared__ double scratch[4];
+  const int b = blockIdx.x;
+  if (!slot_active(d, b)) return;
+  const int NN = d.n * d.n;
+  const float* I = d.intensity + (size_t)b * NN;
+  const float* D = d.patterns_s + (size_t)d.group_pos[b] * NN;
+  double model = 0.0, ex = 0.0;
+  for (int e = threadIdx.x; e < NN; e += 256) {
+    model += (double)I[e];
+    ex += (double)D[e];
+  }
+  model = block_sum_f64(model, scratch);
+  ex = block_sum_f64(ex, scratch);
+  if (threadIdx.x == 0) d.errors[b] = (float)ex / (float)model;
+}
+
 // out[pos[b]][natural k] = intensity[b][scrambled k]  (model intensity of each slot)
 template <int N>
 __global__ void exit_intensity_copy_k(phx_group_desc d, float* out) {
@@ -1616,7 +1676,7 @@ int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe,
   if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // needs every psi_i
   const bool p0 = want_probe || want_positions;
 #define K(N, T, WO, WP) \
-  hipLaunchKernelGGL((gradient_sweep_k<N, T, WO, WP>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, want_probe, \
+  hipLaunchKernelGGL((gradient_sweep_k<N, T, WO, WP, 0>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, want_probe, \
                      want_positions)
 #define L(N, T)                 \
   do {                          \
@@ -1631,6 +1691,45 @@ int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe,
   } while (0)
   PHX_DISPATCH(d->n, L);
 #undef L
+#undef K
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_gradient_sweep_farfield(const phx_group_desc* d, int poisson, int want_object, int want_probe,
+                                int want_positions, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->chi_a || !d->farfield || !d->intensity || !d->patterns_s || !d->mask_s ||
+      (d->n_slices > 1 && !d->props) || (want_object && d->n_slices > 1 && !d->chi_slices) ||
+      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)))
+    return PHX_EINVAL;
+  if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // needs every psi_i
+  const bool p0 = want_probe || want_positions;
+#define K(N, T, WO, WP, FF) \
+  hipLaunchKernelGGL((gradient_sweep_k<N, T, WO, WP, FF>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, want_probe, \
+                     want_positions)
+#define KF(N, T, WO, WP)                 \
+  do {                                   \
+    if (poisson)                         \
+      K(N, T, WO, WP, FM_GRAD_POISSON);  \
+    else                                 \
+      K(N, T, WO, WP, FM_GRAD_AMPLITUDE);\
+  } while (0)
+#define L(N, T)                 \
+  do {                          \
+    if (want_object && p0)      \
+      KF(N, T, true, true);     \
+    else if (want_object)       \
+      KF(N, T, true, false);    \
+    else if (p0)                \
+      KF(N, T, false, true);    \
+    else                        \
+      KF(N, T, false, false);   \
+  } while (0)
+  PHX_DISPATCH(d->n, L);
+#undef L
+#undef KF
 #undef K
   PHX_CHECK_LAUNCH();
   return PHX_OK;
@@ -1829,6 +1928,15 @@ int phx_dry_run_stats_gradient(const phx_group_desc* d, hipStream_t stream) {
   if (r) return r;
   if (!d->patterns_s || !d->errors) return PHX_EINVAL;
   hipLaunchKernelGGL(dry_run_stats_gradient_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_dry_run_stats_intensity(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->patterns_s || !d->errors || !d->intensity) return PHX_EINVAL;
+  hipLaunchKernelGGL(dry_run_stats_intensity_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

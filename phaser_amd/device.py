@@ -42,11 +42,24 @@ def stream_handle():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def host_patterns(patterns):
+    """Diffraction patterns as given to an engine: a torch tensor already on the GPU is
+    kept there (no host round trip); anything else becomes a numpy array."""
+    if hasattr(patterns, 'is_cuda') and patterns.is_cuda:
+        return patterns
+    return numpy.asarray(patterns)
+
+
 def to_device_scrambled(torch, arr, n, is_complex, device):
-    """Upload (batch, n, n) natural-order frequency data and scramble it on the device."""
-    arr = numpy.ascontiguousarray(arr)
-    batch = arr.reshape(-1, n, n).shape[0]
-    src = torch.from_numpy(arr.reshape(batch, n * n)).to(device)
+    """Upload (batch, n, n) natural-order frequency data and scramble it on the device
+    (a device tensor is scrambled where it is)."""
+    if hasattr(arr, 'is_cuda') and arr.is_cuda:
+        batch = arr.reshape(-1, n, n).shape[0]
+        src = arr.reshape(batch, n * n).contiguous()
+    else:
+        arr = numpy.ascontiguousarray(arr)
+        batch = arr.reshape(-1, n, n).shape[0]
+        src = torch.from_numpy(arr.reshape(batch, n * n)).to(device)
     out = torch.empty_like(src)
     st = stream_handle()
     for b0 in range(0, batch, 65535):
@@ -92,8 +105,10 @@ class GroupWorkspace:
                                       "16, 32, 64, 128 or 192")
         self.n = n = ny
         self.NN = n * n
-        pats = numpy.asarray(patterns)
-        self.scan_shape = tuple(pats.shape[:-2])
+        pats = host_patterns(patterns)
+        if hasattr(pats, 'is_cuda') and pats.dtype != torch.float32:
+            raise TypeError("device patterns must be float32")
+        self.scan_shape = tuple(int(v) for v in pats.shape[:-2])
         self.P = int(numpy.prod(self.scan_shape))
         self.G, self.B = int(n_groups), int(bmax)
         f32, c64, i32 = torch.float32, torch.complex64, torch.int32
@@ -105,7 +120,8 @@ class GroupWorkspace:
         self.ky_s = torch.from_numpy(numpy.ascontiguousarray(ky[f])).to(dev)
         self.kx_s = torch.from_numpy(numpy.ascontiguousarray(kx[f])).to(dev)
         self.twiddle = torch.from_numpy(twiddles(n)).to(dev)
-        self.patterns_s = to_device_scrambled(torch, pats.astype(numpy.float32), n, False, dev)
+        self.patterns_s = to_device_scrambled(torch, pats if hasattr(pats, 'is_cuda') else pats.astype(numpy.float32),
+                                              n, False, dev)
         self.mask_s = to_device_scrambled(torch, numpy.asarray(pattern_mask, dtype=numpy.float32), n, False, dev).reshape(-1)
         self.props_s = None if props is None else to_device_scrambled(torch, props.astype(numpy.complex64), n, True, dev)
         self.prop_zero_waves = zero_column_waves(props, n)

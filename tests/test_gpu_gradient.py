@@ -116,6 +116,51 @@ def test_recompute_equals_store():
         assert e < 1e-5
 
 
+@pytest.mark.parametrize('noise', ['amplitude', 'poisson'])
+def test_farfield_step_in_sweep_equals_separate_launch(noise):
+    """The far-field gradient step and the loss computed in the adjoint sweep's first pass
+    (phx_gradient_sweep_farfield) against phx_farfield_gradient_update followed by
+    phx_gradient_sweep: the same values (object gradient bit-identical; the loss and the
+    probe/position gradients are summed with atomics)."""
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import AmplitudeNoisePlan, PoissonNoisePlan
+    _cfg, state, pats = _problem(S=4, M=2, noise=noise)
+    group = numpy.array(numpy.unravel_index(numpy.arange(25), (5, 5)))
+    nplan = PoissonNoisePlan() if noise == 'poisson' else AmplitudeNoisePlan()
+    res = []
+    for deferred in (False, True):
+        model, _ = make_model(state, pats.patterns, pats.pattern_mask, nplan, groups=[group])
+        assert model.fused
+        model.forward(0, loss_in_backward=deferred)
+        model.backward(0)
+        res.append((model.ws.grad_obj.cpu().numpy(), model.ws.grad_probe.cpu().numpy(),
+                    model.position_grads(0).cpu().numpy(), float(model.loss_tensor(0))))
+    numpy.testing.assert_array_equal(res[0][0], res[1][0])
+    for a, b in zip(res[0][1:3], res[1][1:3]):
+        assert rel_err(a, b) < 1e-6
+    assert abs(res[0][3] - res[1][3]) <= 1e-9 * abs(res[0][3])
+
+
+def test_dry_run_from_forward_intensity_equals_per_slice():
+    """The gradient engine's dry-run ratios sum(D) / sum_m |psi_{S-1} O_{S-1}|^2 from one
+    fused forward launch and its stored intensity (Parseval) against the per-slice
+    kernels that sum |psi O|^2 directly."""
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import AmplitudeNoisePlan
+    _cfg, state, pats = _problem(S=4, M=2)
+    group = numpy.array(numpy.unravel_index(numpy.arange(25), (5, 5)))
+    model, _ = make_model(state, pats.patterns, pats.pattern_mask, AmplitudeNoisePlan(), groups=[group])
+    assert model.fused
+    out = []
+    for fused in (True, False):
+        model.fused = fused
+        model.dry_run(0)
+        out.append(model.ws.errors[0, :25].cpu().numpy().copy())
+    model.fused = True
+    print('dry run ratios', out[0][:4], 'rel', rel_err(out[0], out[1]), flush=True)
+    assert rel_err(out[0], out[1]) < 1e-5
+
+
 def test_autograd_function_backward():
     import torch
     from phaser_amd.engines.gradient import make_model, multislice_loss
