@@ -157,6 +157,45 @@ def cpu_baseline(state, pats, groups, niter_groups=1):
     return n / dt, n, dt
 
 
+def engines_timing(niter):
+    """Wall time of the gradient engine (SGD with momentum on the object, the learned
+    comparison's settings) and of plain LSQ-ML on the time-to-NRMSE problem, for 1 and
+    `niter` iterations each after one warm-up run; patterns resident in HBM.  A short
+    workload to profile the two engines side by side."""
+    import torch
+    from dataclasses import replace
+    from phaser_amd import learned
+    from phaser_amd.engines import gradient as geng
+    from phaser_amd.engines.conventional import run_engine as conv_run
+    from phaser_amd.plan import GradientEnginePlan, ConventionalEnginePlan
+    from phaser_amd.synthetic import make_dataset
+    state0, pats, _truth = make_dataset(learned.evaluation_config())
+    pats = replace(pats, patterns=torch.from_numpy(numpy.ascontiguousarray(pats.patterns)).cuda())
+
+    def grad(n):
+        p = GradientEnginePlan(noise_model='amplitude', niter=n, grouping=64,
+                               solvers={'object': {'type': 'sgd', 'learning_rate': 30.0, 'momentum': 0.9}})
+        geng.run_engine({'data': pats, 'state': state0.copy().to_device(), 'dtype': numpy.float32, 'seed': 0}, p)
+
+    def plain(n):
+        p = ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=n, grouping=64)
+        conv_run({'data': pats, 'state': state0.copy(), 'dtype': numpy.float32, 'seed': 0}, p)
+    grad(1)
+    plain(1)
+    torch.cuda.synchronize()
+    for name, f in (('gradient', grad), ('plain LSQ-ML', plain)):
+        t = {}
+        for n in (1, niter):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            f(n)
+            torch.cuda.synchronize()
+            t[n] = time.perf_counter() - t0
+            log(f"{name}, {n} iteration(s): {t[n]:.4f} s")
+        if niter > 1:
+            log(f"{name}: {(t[niter] - t[1]) / (niter - 1) * 1e3:.2f} ms per iteration")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +210,13 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a device buffer of this many MiB once (a copy kernel of "
                          "known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--engines-timing", type=int, default=0, metavar="N",
+                    help="only time the gradient engine and plain LSQ-ML on the time-to-NRMSE problem "
+                         "(1 and N iterations each) and exit")
     args = ap.parse_args()
+    if args.engines_timing:
+        engines_timing(args.engines_timing)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
               "build", file=sys.stderr)
