@@ -478,8 +478,7 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
     if (inside) step(std::true_type{});
     else step(std::false_type{});
     if (i == S - 1) break;
-    const float2* prop = (const float2*)d.props + (size_t)i * NN;
-    pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); }, d.prop_zero_waves);
+    pl.template convolve_midfetch<false>(x, (const float2*)d.props + (size_t)i * NN, d.prop_zero_waves);
   }
   pl.forward_regs(x);
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;

@@ -523,6 +523,27 @@ struct Plane {
     cols_to_rows(x);
     inv<false>(x);
   }
+  // as convolve_prefetched, the multiplier's loads issued after the row -> column
+  // transpose instead: in flight during the forward column pass only
+  template <bool CONJ>
+  __device__ __forceinline__ void convolve_midfetch(float2 (&x)[E], const float2* mul, unsigned zero_waves = 0) {
+    fwd<false>(x);
+    rows_to_cols(x);
+    if (wave_is_zero(zero_waves)) {
+      zero_column_pass(x);
+    } else {
+      float2 mv[E];
+#pragma unroll
+      for (int k = 0; k < E; ++k) mv[k] = mul[(QL * k + q) * N + rc];
+      fwd<true>(x);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < E; ++k) x[k] = CONJ ? cmulc(x[k], mv[k]) : cmul(x[k], mv[k]);
+      inv<true>(x);
+    }
+    cols_to_rows(x);
+    inv<false>(x);
+  }
   template <class Src, class Mul, class Dst, class Pin>
   __device__ __forceinline__ void convolve(Src src, Mul mul, Dst dst, Pin pin) {
     float2 x[E];
