@@ -75,7 +75,8 @@ def fft_flops(nn):
 
 def fft_microbench(torch, n=128, batch=2048, reps=5):
     """Batched forward fft2 (phx_fft2_c64) of `batch` n x n complex64 planes: average
-    launch time over `reps` launches (HIP events on the launch stream)."""
+    call time over `reps` calls (HIP events on the launch stream; at n = 256 a call is
+    two launches).  `gbs` counts one read and one write of every plane."""
     import ctypes
     from phaser_amd import _lib
     from phaser_amd.optics import twiddles
@@ -304,7 +305,11 @@ def main():
     fft = {"whole_iteration_tflops": fft_tf, "peak_tflops": FP32_VECTOR_PEAK_TFLOPS,
            "whole_iteration_frac": fft_tf / FP32_VECTOR_PEAK_TFLOPS,
            "transforms_per_pattern_iter": 4 * M * S,
-           "batched_fft2": fft_microbench(torch, n=int(round(NN ** 0.5)))}
+           "batched_fft2": fft_microbench(torch, n=int(round(NN ** 0.5))),
+           # every plane side the path supports, 256 MiB of planes each (n = 256: the
+           # split-plane transform, a row launch and an in-place column launch)
+           "batched_fft2_by_n": [fft_microbench(torch, n=k, batch=(256 << 20) // (8 * k * k))
+                                 for k in (64, 128, 192, 256)]}
     log(f"fft rate: {json.dumps(fft)}")
 
     cpu = None

@@ -50,8 +50,9 @@ int phx_fill_u32(void* ptr, uint32_t value, int64_t count, hipStream_t stream);
 
 /* Batched fft2 (inverse=0) / ifft2 (inverse=1) with phaser's convention
  * (phaser/utils/num.py:434-468: centred real space, corner-origin reciprocal space,
- * norm='ortho') over `batch` n x n planes, n in {16, 32, 64, 128, 192}.  `twiddle` holds the
- * n complex64 values exp(-2 pi i j / n).  in != out. */
+ * norm='ortho') over `batch` n x n planes, n in {16, 32, 64, 128, 192, 256}.  `twiddle` holds
+ * the n complex64 values exp(-2 pi i j / n).  in != out.  n = 256 runs as two launches (rows
+ * into `out`, then columns in place in `out`) and takes batch <= 2^20. */
 int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int inverse, const void* twiddle,
                  hipStream_t stream);
 

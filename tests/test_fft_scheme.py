@@ -153,3 +153,46 @@ def test_pointwise_between_transforms_equals_fft_convolution():
 def test_unsupported_side_raises():
     with pytest.raises(ValueError):
         scr_to_freq(48)
+
+
+def _line256(x, inverse=False):
+    """One line of fft256_pass_k (phx_cutout.hip): thread t holds x[16 a + t] and runs a
+    DFT_16 over a, the twiddle W_256^(t k1), writes the block slot [k1][t]; thread t then
+    plays k1 = t, reads slots [t][j] over j and runs a second DFT_16, whose register k2
+    is X[k1 + 16 k2]."""
+    sg = 1.0 if inverse else -1.0
+    dft16 = (lambda v: numpy.fft.ifft(v) * 16) if inverse else numpy.fft.fft
+    tw = numpy.exp(sg * 2j * numpy.pi * numpy.arange(256) / 256)
+    blk = numpy.zeros((16, 16), complex)
+    for t in range(16):
+        y = dft16(x[16 * numpy.arange(16) + t])
+        y = y * tw[(t * numpy.arange(16)) & 255]
+        blk[:, t] = y                      # slot k1 * 17 + t
+    out = numpy.zeros(256, complex)
+    for t in range(16):                    # thread t as k1
+        z = dft16(blk[t, :])
+        out[t + 16 * numpy.arange(16)] = z
+    return out
+
+
+def _fft2_256(a, inverse=False):
+    """Row pass then column pass, with the phaser-convention sign and 1/n scale placed
+    where the kernel places them (row-pass input for ifft2, column-pass output for fft2)."""
+    n = 256
+    ky, kx = numpy.meshgrid(numpy.arange(n), numpy.arange(n), indexing='ij')
+    sgn = numpy.where((ky + kx) & 1, -1.0, 1.0) / n
+    a = a * sgn if inverse else a
+    r = numpy.stack([_line256(a[y], inverse) for y in range(n)])
+    c = numpy.stack([_line256(r[:, x], inverse) for x in range(n)]).T
+    return c if inverse else c * sgn
+
+
+def test_fft256_split_plane_scheme_matches_phaser_convention():
+    import oracle.num as onum
+    rng = numpy.random.default_rng(256)
+    x = rng.normal(size=256) + 1j * rng.normal(size=256)
+    numpy.testing.assert_allclose(_line256(x), numpy.fft.fft(x), atol=1e-10)
+    numpy.testing.assert_allclose(_line256(x, True), numpy.fft.ifft(x) * 256, atol=1e-10)
+    a = rng.normal(size=(256, 256)) + 1j * rng.normal(size=(256, 256))
+    numpy.testing.assert_allclose(_fft2_256(a), onum.fft2(a), atol=1e-10)
+    numpy.testing.assert_allclose(_fft2_256(a, True), onum.ifft2(a), atol=1e-10)

@@ -22,7 +22,7 @@ def _st():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-@pytest.mark.parametrize('n', [16, 32, 64, 128, 192])
+@pytest.mark.parametrize('n', [16, 32, 64, 128, 192, 256])
 def test_fft2_ifft2_match_oracle(n):
     import torch
     from phaser_amd import _lib
@@ -172,3 +172,24 @@ def test_scramble_roundtrip():
         numpy.testing.assert_array_equal(out.cpu().numpy(), scramble(a))
         _lib.call('phx_scramble', _ptr(out), _ptr(back), 2, n, 0, 1, _st())
         numpy.testing.assert_array_equal(back.cpu().numpy(), a)
+
+
+def test_fft256_roundtrip_and_parseval_at_batch_64():
+    """n = 256 (two launches, the column pass in place): ifft2(fft2(a)) = a and
+    sum |fft2(a)|^2 = sum |a|^2 (norm='ortho') over 64 planes."""
+    import torch
+    from phaser_amd import _lib
+    from phaser_amd.optics import twiddles
+    n, batch = 256, 64
+    g = torch.Generator().manual_seed(256)
+    a = torch.complex(torch.randn(batch, n, n, generator=g), torch.randn(batch, n, n, generator=g)).cuda()
+    f = torch.empty_like(a)
+    back = torch.empty_like(a)
+    tw = torch.from_numpy(twiddles(n)).cuda()
+    _lib.call('phx_fft2_c64', _ptr(a), _ptr(f), batch, n, 0, _ptr(tw), _st())
+    _lib.call('phx_fft2_c64', _ptr(f), _ptr(back), batch, n, 1, _ptr(tw), _st())
+    torch.cuda.synchronize()
+    assert float((back - a).abs().max()) < 1e-5
+    ea = a.abs().square().double().sum(dim=(1, 2))
+    ef = f.abs().square().double().sum(dim=(1, 2))
+    assert float(((ef - ea) / ea).abs().max()) < 1e-6
