@@ -151,7 +151,7 @@ int launch_fft2(const void* in, void* out, int64_t batch, int inverse, const voi
 // A 256 x 256 complex64 plane (512 KiB) fits neither a CU's LDS nor one workgroup's
 // registers (64 points per lane at 1024 threads), so the transform is split: a row
 // pass (in -> out) and a column pass (out -> out, in place), each a launch of
-// workgroups of 256 threads that own 16 whole lines of one plane.  A line is held by
+// workgroups that own 16 (rows) or 32 (columns) whole lines of one plane.  A line is held by
 // 16 threads; thread t of a line loads x[16 a + t], a < 16, and the length-256
 // transform is the four-step 16 x 16 split
 //   X[k1 + 16 k2] = sum_t W_16^(t k2) W_256^(t k1) sum_a x[16 a + t] W_16^(a k1):
@@ -159,60 +159,66 @@ int launch_fft2(const void* in, void* out, int64_t batch, int inverse, const voi
 // through LDS after which thread k1 holds the 16 values of index k1, and a second
 // in-register DFT_16 that leaves X[k1 + 16 k2] in register k2.  The row pass reads and
 // writes whole rows (a wave covers 4 rows, 512 contiguous bytes per load); the column
-// pass reads and writes 16-column strips (a wave covers 4 rows of 128 bytes).  The
+// pass reads and writes 32-column strips (a wave covers 2 rows of 256 bytes).  The
 // column pass reads its strip whole before writing it, and no other workgroup touches
 // it, so in place is safe.  Traffic: 2 x 512 KiB read + 2 x 512 KiB written per plane.
-constexpr int F256_LD = 17;              // padded stride of a line's 16 x 16 exchange block
-constexpr int F256_BLK = 16 * F256_LD;   // one line's block
+constexpr int F256_LD = 17;  // padded row of a line's 16 x 16 exchange block
 
-template <bool COL>
-__global__ void __launch_bounds__(256) fft256_pass_k(const float2* in, float2* out,
-                                                      const float2* __restrict__ twg, int inverse) {
-  __shared__ float2 ex[16 * F256_BLK];
+// LINES lines per workgroup of 16 LINES threads: 16 rows in the row pass, 32 columns in
+// the column pass (a wave then reads 2 rows x 256 contiguous bytes per load).
+template <bool COL, int LINES, bool INV>
+__global__ void __launch_bounds__(16 * LINES) fft256_pass_k(const float2* in, float2* out,
+                                                             const float2* __restrict__ twg) {
+  constexpr int TILES = 256 / LINES;
+  // stride between the lines' blocks: in the column pass the LINES lines of a load are
+  // adjacent lanes, so an odd stride spreads them over the banks; in the row pass a
+  // wave's 4 lines at stride 272 (16 mod 32 float2 slots) meet at most 2 to a bank
+  constexpr int BLK = 16 * F256_LD + (COL ? 1 : 0);
+  __shared__ float2 ex[LINES * BLK];
   __shared__ float2 tw[256];
   const int tid = threadIdx.x;
-  const int tile = blockIdx.x & 15;
-  const size_t plane = (size_t)(blockIdx.x >> 4) * 256 * 256;
-  // row pass: line l = tid / 16 is row 16 tile + l; column pass: line l = tid % 16 is
-  // column 16 tile + l (16 adjacent threads read adjacent columns of one row)
-  const int l = COL ? (tid & 15) : (tid >> 4);
-  const int t = COL ? (tid >> 4) : (tid & 15);
-  const int line = 16 * tile + l;
+  const int tile = blockIdx.x % TILES;
+  const size_t plane = (size_t)(blockIdx.x / TILES) * 256 * 256;
+  // row pass: line l = tid / 16 is row LINES tile + l; column pass: line l = tid % LINES
+  // is column LINES tile + l (LINES adjacent threads read adjacent columns of one row)
+  const int l = COL ? (tid % LINES) : (tid / 16);
+  const int t = COL ? (tid / LINES) : (tid % 16);
+  const int line = LINES * tile + l;
   // element e of this line sits at base + e * es
   const size_t base = plane + (COL ? (size_t)line : (size_t)line * 256);
   const int es = COL ? 256 : 1;
-  tw[tid] = twg[tid];
+  for (int k = tid; k < 256; k += 16 * LINES) tw[k] = twg[k];
   const float sc = 1.0f / 256.0f;
   float2 x[16];
 #pragma unroll
   for (int a = 0; a < 16; ++a) {
     float2 v = in[base + (size_t)(16 * a + t) * es];
-    if (!COL && inverse) {
+    if (!COL && INV) {
       // ifft2(A) = IFFT((-1)^(ky+kx) A) / n: the row pass sees (ky, kx) = (line, e)
       v = cscale(v, ((line + 16 * a + t) & 1) ? -sc : sc);
     }
     x[a] = v;
   }
   __syncthreads();  // twiddle table written
-  if (inverse) dft<16, true>(x); else dft<16, false>(x);
+  dft<16, INV>(x);
 #pragma unroll
   for (int k1 = 1; k1 < 16; ++k1) {
     const float2 w = tw[(t * k1) & 255];
-    x[k1] = inverse ? cmulc(x[k1], w) : cmul(x[k1], w);
+    x[k1] = INV ? cmulc(x[k1], w) : cmul(x[k1], w);
   }
-  float2* blk = ex + l * F256_BLK;
+  float2* blk = ex + l * BLK;
 #pragma unroll
   for (int k1 = 0; k1 < 16; ++k1) blk[k1 * F256_LD + t] = x[k1];
   __syncthreads();
   // thread t now plays k1 = t
 #pragma unroll
   for (int j = 0; j < 16; ++j) x[j] = blk[t * F256_LD + j];
-  if (inverse) dft<16, true>(x); else dft<16, false>(x);
+  dft<16, INV>(x);
 #pragma unroll
   for (int k2 = 0; k2 < 16; ++k2) {
     const int e = t + 16 * k2;
     float2 v = x[k2];
-    if (COL && !inverse) {
+    if (COL && !INV) {
       // fft2(a) = (-1)^(ky+kx) FFT(a) / n: the column pass sees (ky, kx) = (e, line)
       v = cscale(v, ((line + e) & 1) ? -sc : sc);
     }
@@ -220,15 +226,24 @@ __global__ void __launch_bounds__(256) fft256_pass_k(const float2* in, float2* o
   }
 }
 
+#ifndef PHX_F256_COL_LINES
+#define PHX_F256_COL_LINES 32
+#endif
+constexpr int F256_ROW_LINES = 16, F256_COL_LINES = PHX_F256_COL_LINES;
+
+template <bool INV>
+void launch_fft256_passes(const void* in, void* out, int64_t batch, const void* tw, hipStream_t stream) {
+  hipLaunchKernelGGL((fft256_pass_k<false, F256_ROW_LINES, INV>), dim3((unsigned)(batch * (256 / F256_ROW_LINES))),
+                     dim3(16 * F256_ROW_LINES), 0, stream, (const float2*)in, (float2*)out, (const float2*)tw);
+  hipLaunchKernelGGL((fft256_pass_k<true, F256_COL_LINES, INV>), dim3((unsigned)(batch * (256 / F256_COL_LINES))),
+                     dim3(16 * F256_COL_LINES), 0, stream, (const float2*)out, (float2*)out, (const float2*)tw);
+}
+
 int launch_fft256(const void* in, void* out, int64_t batch, int inverse, const void* tw, hipStream_t stream) {
-  // 16 workgroups of 256 threads per plane; the grid's x extent bounds the batch
+  // 256 / LINES workgroups per plane and pass; the grid's x extent bounds the batch
   if (batch > (1LL << 20)) return PHX_EINVAL;
-  const dim3 grid((unsigned)(batch * 16));
-  hipLaunchKernelGGL((fft256_pass_k<false>), grid, dim3(256), 0, stream, (const float2*)in, (float2*)out,
-                     (const float2*)tw, inverse);
-  PHX_CHECK_LAUNCH();
-  hipLaunchKernelGGL((fft256_pass_k<true>), grid, dim3(256), 0, stream, (const float2*)out, (float2*)out,
-                     (const float2*)tw, inverse);
+  if (inverse) launch_fft256_passes<true>(in, out, batch, tw, stream);
+  else launch_fft256_passes<false>(in, out, batch, tw, stream);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }
