@@ -198,6 +198,14 @@ int phx_lsqml_object_update(const phx_group_desc* d, int slice, hipStream_t stre
 /* The updates of slices first .. first+count-1 in one launch (needs chi_slices when
  * count > 1). */
 int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count, hipStream_t stream);
+/* phx_lsqml_object_update_slices(d, first, count) and phx_group_begin(next) in one launch:
+ * the update of group d also zeroes the accumulators of the group `next` starts from (the
+ * fresh per-group arrays of lsqml_run, solvers.py:171-200).  Valid when d's update reads
+ * none of next's buffers: next->red must differ from d->red; gom, intensity and errors are
+ * not read by the update.  Saves the separate clear launch between a group's backward
+ * sweep and its object update. */
+int phx_lsqml_object_update_slices_begin(const phx_group_desc* d, int first, int count, const phx_group_desc* next,
+                                         hipStream_t stream);
 /* d_probe[m] = sum_b farfield[b][m] (position order): the probe-update spectrum. */
 int phx_probe_spectrum_reduce(const phx_group_desc* d, hipStream_t stream);
 /* P += beta_P (sum_b alpha_P) ifft2(d_probe)/(gom + reg) * gom/(gom + 1e-16); d_probe = 0;

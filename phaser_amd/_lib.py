@@ -79,6 +79,7 @@ SIGNATURES = {
     "phx_stitch_patches": (c_int, [c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp,
                                    c_vp]),
     "phx_lsqml_object_update_slices": (c_int, [_P, c_int, c_int, c_vp]),
+    "phx_lsqml_object_update_slices_begin": (c_int, [_P, c_int, c_int, _P, c_vp]),
     "phx_probe_spectrum_reduce": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply_strips": (c_int, [_P, c_vp, c_vp]),

@@ -857,6 +857,32 @@ __device__ __forceinline__ void gather_tile(const phx_group_desc& d, int i, int 
   }
 }
 
+// Accumulators a group step starts from zero (gom, intensity, red, errors); cleared by
+// zero4_k, or by the previous group's object update in the same launch.
+struct ClearSpan {
+  float* a;
+  int64_t na;
+  float* b;
+  int64_t nb;
+  double* c;
+  int64_t nc;
+  float* e;
+  int64_t ne;
+};
+
+// Grid-stride zeroing of a ClearSpan by every thread of the launch (any grid shape).
+__device__ __forceinline__ void clear_span(const ClearSpan& z) {
+  const int64_t nthr = (int64_t)gridDim.x * gridDim.y * gridDim.z * blockDim.x;
+  const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const int64_t tot = z.na + z.nb + z.nc + z.ne;
+  for (int64_t k = blk * blockDim.x + threadIdx.x; k < tot; k += nthr) {
+    if (k < z.na) z.a[k] = 0.f;
+    else if (k < z.na + z.nb) z.b[k - z.na] = 0.f;
+    else if (k < z.na + z.nb + z.nc) z.c[k - z.na - z.nb] = 0.0;
+    else z.e[k - z.na - z.nb - z.nc] = 0.f;
+  }
+}
+
 // LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
 //   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
 //   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
@@ -867,7 +893,10 @@ __device__ __forceinline__ void gather_tile(const phx_group_desc& d, int i, int 
 // prefix count); then every thread gathers both sums from those positions' chi_i and
 // psi_i planes (gather_tile): no atomics, and the result does not depend on scheduling.
 // Pixels no cutout covers are left untouched (their reference update is exactly zero).
-__global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int first) {
+// `z` (possibly empty) is the next group's accumulator clear, issued first: the update
+// reads none of those buffers (its own red slot is not the next group's).
+__global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, int first, ClearSpan z) {
+  clear_span(z);
   const int i = first + (int)blockIdx.z;
   __shared__ float sA;
   const RedLayout L = red_layout(d.n_slices, d.bmax, d.n_modes);
@@ -1420,17 +1449,23 @@ __global__ void position_finish_k(phx_group_desc d, float* pos_update) {
     pos_update[2 * p + dir] = (float)(d.red[L.pos_num + 2 * b + dir] / d.red[L.pos_den + dir]);
 }
 
-__global__ void zero4_k(float* a, int64_t na, float* b, int64_t nb, double* c, int64_t nc, float* e, int64_t ne) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < na + nb + nc + ne; k += stride) {
-    if (k < na) a[k] = 0.f;
-    else if (k < na + nb) b[k - na] = 0.f;
-    else if (k < na + nb + nc) c[k - na - nb] = 0.0;
-    else e[k - na - nb - nc] = 0.f;
-  }
-}
+__global__ void zero4_k(ClearSpan z) { clear_span(z); }
 
 // --------------------------------- host side ---------------------------------------
+// The accumulators phx_group_begin clears for group descriptor d.
+ClearSpan begin_span(const phx_group_desc& d) {
+  ClearSpan z;
+  z.a = d.gom;
+  z.na = (int64_t)d.n * d.n;
+  z.b = d.intensity;
+  z.nb = (int64_t)d.bmax * d.n * d.n;
+  z.c = d.red;
+  z.nc = red_layout(d.n_slices, d.bmax, d.n_modes).total;
+  z.e = d.errors;
+  z.ne = d.bmax;
+  return z;
+}
+
 bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192; }
 
 int check_desc(const phx_group_desc* d) {
@@ -1489,14 +1524,10 @@ int phx_group_begin(const phx_group_desc* d, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
   if (!d->gom || !d->intensity || !d->errors) return PHX_EINVAL;
-  const int64_t na = (int64_t)d->n * d->n;
-  const int64_t nb = (int64_t)d->bmax * d->n * d->n;
-  const int64_t nc = red_layout(d->n_slices, d->bmax, d->n_modes).total;
-  const int64_t ne = d->bmax;
-  int64_t blocks = (na + nb + nc + ne + 255) / 256;
+  const ClearSpan z = begin_span(*d);
+  int64_t blocks = (z.na + z.nb + z.nc + z.ne + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(zero4_k, dim3((unsigned)blocks), dim3(256), 0, stream, d->gom, na, d->intensity, nb, d->red, nc,
-                     d->errors, ne);
+  hipLaunchKernelGGL(zero4_k, dim3((unsigned)blocks), dim3(256), 0, stream, z);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }
@@ -1742,7 +1773,24 @@ int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count
     return PHX_EINVAL;
   const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH),
                   (unsigned)count);
-  hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, first);
+  hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, first, ClearSpan{});
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_lsqml_object_update_slices_begin(const phx_group_desc* d, int first, int count, const phx_group_desc* next,
+                                         hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  r = check_desc(next);
+  if (r) return r;
+  if (first < 0 || count < 1 || first + count > d->n_slices || !d->chi_a || !d->chi_b || !d->finite_flag ||
+      (count > 1 && !d->chi_slices))
+    return PHX_EINVAL;
+  if (!next->gom || !next->intensity || !next->errors || next->red == d->red) return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH),
+                  (unsigned)count);
+  hipLaunchKernelGGL(lsqml_object_update_k, grid, dim3(256), 0, stream, *d, first, begin_span(*next));
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

@@ -331,15 +331,21 @@ class GroupWorkspace:
             if side is not None:
                 side.wait_stream(main)
             self._begun = None
-            if g + 1 < self.G:
+            # with the strip probe step and an object update, the update launch itself
+            # clears group g+1's accumulators (none of which it reads)
+            fold_begin = self.probe_strips and update_object and g + 1 < self.G
+            if g + 1 < self.G and not fold_begin:
                 c("phx_group_begin", self.desc_refs[g + 1], st)
                 self._begun = g + 1
             if self.probe_strips:
                 # the probe step in strips of small workgroups, which run beside the object
                 # update as its workgroups retire: the update starts right away
-                if update_object:
+                if fold_begin:
                     # after the position step, which reads the pre-update slice-0 cutout
                     # (solvers.py:275-288)
+                    c("phx_lsqml_object_update_slices_begin", d, 0, self.S, self.desc_refs[g + 1], st)
+                    self._begun = g + 1
+                elif update_object:
                     c("phx_lsqml_object_update_slices", d, 0, self.S, st)
                 if update_probe:
                     c("phx_probe_spectrum_reduce", d, sh)
