@@ -24,6 +24,9 @@ from .optics import fft_lanes, scr_to_freq, twiddles
 INT32_MAX = 2**31 - 1
 # largest plane whose LSQ-ML probe step runs in strips (phx_lsqml_probe_apply_strips)
 PROBE_STRIPS_MAX_N = 128
+# the object-update launch of group g also clears group g+1's accumulators
+# (phx_lsqml_object_update_slices_begin) instead of a separate phx_group_begin
+FOLD_GROUP_BEGIN = True
 
 
 def _require_torch_gpu():
@@ -333,7 +336,7 @@ class GroupWorkspace:
             self._begun = None
             # with the strip probe step and an object update, the update launch itself
             # clears group g+1's accumulators (none of which it reads)
-            fold_begin = self.probe_strips and update_object and g + 1 < self.G
+            fold_begin = FOLD_GROUP_BEGIN and self.probe_strips and update_object and g + 1 < self.G
             if g + 1 < self.G and not fold_begin:
                 c("phx_group_begin", self.desc_refs[g + 1], st)
                 self._begun = g + 1

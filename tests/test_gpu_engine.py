@@ -313,3 +313,34 @@ def test_fused_sweeps_and_deferred_updates_equal_per_slice_launches(solver, slic
         # which moves the next iteration's error at the 1e-6 level
         numpy.testing.assert_allclose(o.progress.detector_errors, ref.progress.detector_errors,
                                       rtol=1e-5 if solver == 'epie' else 1e-6)
+
+
+def test_update_launch_clearing_next_group_matches_separate_clear(monkeypatch):
+    """The object update that also zeroes the next group's accumulators
+    (phx_lsqml_object_update_slices_begin) against phx_group_begin launched on its own
+    before each group, with graphs (probe update switching on after the first
+    iteration) and without: equal to the run-to-run noise of the path (atomic sums)."""
+    import phaser_amd.device as dev
+    from phaser_amd.engines.conventional import run_engine
+    from phaser_amd.plan import ConventionalEnginePlan, SOLVERS
+    _, state, pats = _problem(n=128, n_modes=2, n_slices=3, scan_shape=(4, 4), scan_step_px=6.0, jitter_px=0.5,
+                              seed=8)
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver=SOLVERS['lsqml'](beta_object=0.5, beta_probe=0.5),
+                                  niter=3, grouping=4, update_probe={'after': 1})
+    args = {'data': pats, 'dtype': numpy.float32, 'seed': 7, 'observer': None, 'recons_name': 't'}
+
+    def rel(x, y):
+        return float(numpy.abs(x - y).max() / numpy.abs(y).max())
+    for graphs in (True, False):
+        monkeypatch.setattr(dev, 'FOLD_GROUP_BEGIN', True)
+        a = run_engine(dict(args, state=state.copy()), plan, use_graphs=graphs).to_numpy()
+        a2 = run_engine(dict(args, state=state.copy()), plan, use_graphs=graphs).to_numpy()
+        monkeypatch.setattr(dev, 'FOLD_GROUP_BEGIN', False)
+        b = run_engine(dict(args, state=state.copy()), plan, use_graphs=graphs).to_numpy()
+        noise = max(rel(a.object.data, a2.object.data), rel(a.probe.data, a2.probe.data))
+        print('graphs', graphs, 'noise', noise, 'fold vs separate', rel(a.object.data, b.object.data),
+              rel(a.probe.data, b.probe.data), flush=True)
+        assert noise < 1e-5
+        assert rel(a.object.data, b.object.data) < 1e-5
+        assert rel(a.probe.data, b.probe.data) < 1e-5
+        numpy.testing.assert_allclose(a.progress.detector_errors, b.progress.detector_errors, rtol=1e-5)
