@@ -98,6 +98,23 @@ __device__ __forceinline__ float2* plane_scratch(const phx_group_desc& d) {
 
 __device__ __forceinline__ bool slot_active(const phx_group_desc& d, int b) { return b < *d.group_size; }
 
+// Plane (b, m) of this workgroup in a sweep launch (grid n_modes x bmax).  Workgroups go
+// to the 8 XCDs round-robin by linear id L, so with bmax a multiple of 8 the planes are
+// placed at L = 8 (M floor(b/8) + m) + b mod 8: every mode of one position runs on the
+// same XCD and the position's object cutouts are fetched into one L2, not M of them.
+__device__ __forceinline__ void sweep_plane(const phx_group_desc& d, int& b, int& m) {
+  const int M = d.n_modes;
+  if ((d.bmax & 7) == 0) {
+    const int L = blockIdx.y * M + blockIdx.x;
+    const int slot = L >> 3;
+    m = slot % M;
+    b = (slot / M) * 8 + (L & 7);
+  } else {
+    m = blockIdx.x;
+    b = blockIdx.y;
+  }
+}
+
 // Offset of the stored wave of slice i, slot b, mode m.  With psi_slots < S only psi_0
 // is kept permanently and slices 1.. alternate between two further planes, so slice i
 // is recomputed from psi_0 when the adjoint needs it.
@@ -437,7 +454,8 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
 // storing every psi_i and the norm sum_m |psi_0|^2 of each slot.
 template <int N, int T>
 __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
-  const int m = blockIdx.x, b = blockIdx.y;
+  int b, m;
+  sweep_plane(d, b, m);
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
   constexpr int NN = N * N;
@@ -508,7 +526,8 @@ constexpr int BK_WRED_SLICES = 32;
 
 template <int N, int T, bool UPD_OBJ, bool PROBE>
 __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, int calc_error) {
-  const int m = blockIdx.x, b = blockIdx.y;
+  int b, m;
+  sweep_plane(d, b, m);
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
   constexpr int NN = N * N;
@@ -667,7 +686,8 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
 // stores slice S-1's wave gradient into chi_a.
 template <int N, int T, bool WANT_OBJ, bool WANT_P0, int FF>
 __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want_probe, int want_pos) {
-  const int m = blockIdx.x, b = blockIdx.y;
+  int b, m;
+  sweep_plane(d, b, m);
   if (!slot_active(d, b)) return;
   PHX_PLANE_SETUP(N, T);
   constexpr int NN = N * N;
