@@ -269,6 +269,12 @@ int phx_gradient_sweep_farfield(const phx_group_desc* d, int poisson, int want_o
  * and mode order (no atomics; deterministic).  Writes every pixel of those slices (zero
  * where no cutout covers it). */
 int phx_gradient_object_gather(const phx_group_desc* d, int first, int count, hipStream_t stream);
+/* phx_gradient_object_gather(d, first, count) and phx_group_begin(next) in one launch:
+ * the gather of group d also zeroes the accumulators of the group `next` starts from
+ * (gradient/run.py:293-349 runs the groups of an iteration in turn).  next->red must
+ * differ from d->red; the gather reads none of next's buffers. */
+int phx_gradient_object_gather_begin(const phx_group_desc* d, int first, int count, const phx_group_desc* next,
+                                     hipStream_t stream);
 /* grad_probe = ifft2(d_probe) (ortho); d_probe = 0. */
 int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream);
 /* Gradient-engine rescale factors into errors[b]: sum(D) / sum |psi_{S-1} O_{S-1}|^2

@@ -952,7 +952,9 @@ __global__ void __launch_bounds__(256) lsqml_object_update_k(phx_group_desc d, i
 // grad_O_i = sum_{b,m} scatter(g_i conj(psi_i)) from the wave gradients g_i that
 // gradient_sweep_k stored as chi_i; tiles and order as lsqml_object_update_k.  Writes
 // every pixel of its tiles (zero where no cutout covers it).
-__global__ void __launch_bounds__(256) gradient_object_gather_k(phx_group_desc d, int first) {
+// `z` (possibly empty) is the next group's accumulator clear, as in lsqml_object_update_k.
+__global__ void __launch_bounds__(256) gradient_object_gather_k(phx_group_desc d, int first, ClearSpan z) {
+  clear_span(z);
   const int i = first + (int)blockIdx.z;
   const int H = d.obj_h, W = d.obj_w;
   const int ty0 = blockIdx.y * UPD_TH, tx0 = blockIdx.x * UPD_TW;
@@ -1823,7 +1825,24 @@ int phx_gradient_object_gather(const phx_group_desc* d, int first, int count, hi
     return PHX_EINVAL;
   const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH),
                   (unsigned)count);
-  hipLaunchKernelGGL(gradient_object_gather_k, grid, dim3(256), 0, stream, *d, first);
+  hipLaunchKernelGGL(gradient_object_gather_k, grid, dim3(256), 0, stream, *d, first, ClearSpan{});
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_gradient_object_gather_begin(const phx_group_desc* d, int first, int count, const phx_group_desc* next,
+                                     hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  r = check_desc(next);
+  if (r) return r;
+  if (first < 0 || count < 1 || first + count > d->n_slices || !d->chi_a || !d->grad_obj ||
+      (d->n_slices > 1 && !d->chi_slices) || (d->psi_slots > 0 && d->psi_slots < d->n_slices))
+    return PHX_EINVAL;
+  if (!next->gom || !next->intensity || !next->errors || next->red == d->red) return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH),
+                  (unsigned)count);
+  hipLaunchKernelGGL(gradient_object_gather_k, grid, dim3(256), 0, stream, *d, first, begin_span(*next));
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

@@ -29,6 +29,9 @@ from .conventional import Observer
 logger = logging.getLogger(__name__)
 
 PER_ITER_VARS = frozenset({'positions', 'tilt'})
+# the object-gradient gather of group g also clears group g+1's accumulators
+# (phx_gradient_object_gather_begin) instead of a separate phx_group_begin
+FOLD_GROUP_BEGIN = True
 
 
 def red_layout(S, B, M):
@@ -56,6 +59,7 @@ class MultisliceModel:
         self.L = red_layout(ws.S, ws.B, ws.M)
         self._pending = None
         self._farfield_pending = False   # far-field step deferred to the adjoint sweep
+        self._begun = None               # group slot whose accumulators a gather cleared
         # whole-sweep kernels need every slice's wave stored (not the recompute mode) and,
         # with several slices, the per-slice wave-gradient planes (chi_slices)
         ok = ws.psi_slots >= ws.S and (ws.S == 1 or ws.chi_slices is not None)
@@ -65,6 +69,7 @@ class MultisliceModel:
         """errors[b] = sum(D) / model intensity of every slot (gradient/run.py:408-438)."""
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         _lib.call("phx_group_begin", d, st)
+        self._begun = None
         _lib.call("phx_probe_spectra", d, st)
         if self.fused:
             # the whole forward model in one launch; its stored intensity gives the sums
@@ -83,7 +88,9 @@ class MultisliceModel:
         (fused path) leaves the far-field gradient step and the loss to the adjoint sweep
         of backward(), which computes them in its first pass."""
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
-        _lib.call("phx_group_begin", d, st)
+        if self._begun != g:
+            _lib.call("phx_group_begin", d, st)
+        self._begun = None
         if probe_spectra or not self.fused:
             _lib.call("phx_probe_spectra", d, st)
         self._farfield_pending = False
@@ -104,9 +111,11 @@ class MultisliceModel:
     def loss_tensor(self, g):
         return self.ws.red[g, self.L['loss']]
 
-    def backward(self, g, want_object=True, want_probe=True, want_positions=True):
+    def backward(self, g, want_object=True, want_probe=True, want_positions=True, next_group=None):
         """Adjoint pass for the group whose forward ran last; gradients land in
-        ws.grad_obj (S, H, W), ws.grad_probe (M, n*n) and ws.red[g, pos_num]."""
+        ws.grad_obj (S, H, W), ws.grad_probe (M, n*n) and ws.red[g, pos_num].  With
+        `next_group` (the slot whose forward comes next), the object-gradient gather also
+        clears that slot's accumulators, and its forward skips phx_group_begin."""
         if self._pending != g:
             raise RuntimeError("backward() must follow forward() of the same group")
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
@@ -119,7 +128,10 @@ class MultisliceModel:
                 self._farfield_pending = False
             else:
                 _lib.call("phx_gradient_sweep", d, int(want_object), int(want_probe), int(want_positions), st)
-            if want_object:
+            if want_object and next_group is not None and next_group != g:
+                _lib.call("phx_gradient_object_gather_begin", d, 0, ws.S, ws.desc_refs[next_group], st)
+                self._begun = next_group
+            elif want_object:
                 _lib.call("phx_gradient_object_gather", d, 0, ws.S, st)
             if want_probe:
                 _lib.call("phx_gradient_probe_finish", d, st)
@@ -385,7 +397,8 @@ def run_engine(args, props, *, recompute=False):
         for g in range(G):
             # the probe spectra change only with the probe
             model.forward(g, probe_spectra=(g == 0 or want_probe), loss_in_backward=True)
-            model.backward(g, want_object=want_obj, want_probe=want_probe, want_positions=want_pos)
+            model.backward(g, want_object=want_obj, want_probe=want_probe, want_positions=want_pos,
+                           next_group=(g + 1 if FOLD_GROUP_BEGIN and g + 1 < G else None))
             bsz = float(ws.sizes_host[g])
             loss_ptr = ctypes.c_void_p(ws.red[g].data_ptr() + 8 * model.L['loss'])
             for s in group_solvers:

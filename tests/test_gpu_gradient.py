@@ -209,3 +209,29 @@ def test_gradient_engine_matches_oracle(solver):
         a, b, c = rel_err(f(g), f(o32)), rel_err(f(g), f(o64)), rel_err(f(o32), f(o64))
         print(f"{solver} {name}: gpu-vs-fp32 {a:.2e} gpu-vs-fp64 {b:.2e} fp32-vs-fp64 {c:.2e}", flush=True)
         assert b <= 3 * c + 1e-4
+
+
+def test_gather_clearing_next_group_matches_separate_clear(monkeypatch):
+    """The object-gradient gather that also zeroes the next group's accumulators
+    (phx_gradient_object_gather_begin) against phx_group_begin before every group (SGD
+    with momentum, 3 groups, 3 iterations): equal to the run-to-run noise of the path
+    (the probe-gradient spectrum is an atomic sum over positions)."""
+    import phaser_amd.engines.gradient as gr
+    from phaser_amd.plan import GradientEnginePlan, SGDSolverPlan
+    _cfg, state, pats = _problem(S=2, M=2, scan=(6, 6), seed=23)
+    state.probe.data = (state.probe.data / numpy.float32(numpy.sqrt(1e5))).astype(numpy.complex64)
+    plan = GradientEnginePlan(noise_model='amplitude',
+                              solvers={('object', 'probe'): SGDSolverPlan(learning_rate=0.5, momentum=0.5)},
+                              niter=3, grouping=12)
+    runs = []
+    for fold in (True, True, False):
+        monkeypatch.setattr(gr, 'FOLD_GROUP_BEGIN', fold)
+        runs.append(gr.run_engine({'data': pats, 'state': state.copy(), 'seed': 3, 'dtype': numpy.float32},
+                                  plan).to_numpy())
+    a, a2, b = runs
+    for name, f in (('object', lambda s: s.object.data), ('probe', lambda s: s.probe.data),
+                    ('loss', lambda s: s.progress.detector_errors)):
+        noise, e = rel_err(f(a), f(a2)), rel_err(f(a), f(b))
+        print(f"{name}: noise {noise:.2e}, fold vs separate {e:.2e}", flush=True)
+        assert noise < 1e-5
+        assert e < 1e-5
