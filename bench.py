@@ -49,6 +49,10 @@ def kernel_bytes(name, B, M, S, NN, HW):
         # all S slices: gather chi_i and psi_i of every slot; read-modify-write of the
         # covered object
         'lsqml_object_update_slices': S * (B * 2 * M * plane + 2 * c * min(HW, B * NN)),
+        # the same, plus the next group's accumulator clear (object magnitude, intensity,
+        # errors; the few FP64 step-size sums are left out)
+        'lsqml_object_update_slices_begin': S * (B * 2 * M * plane + 2 * c * min(HW, B * NN)) + r * NN * (B + 1)
+        + r * B,
         # fused forward: probe spectra in; every psi_i and the exit spectrum out; the S
         # object cutouts and intensity per position; the propagators
         'group_forward': B * (M * (S + 1) * plane + S * plane + r * NN) + M * plane + (S - 1) * plane,
@@ -284,7 +288,8 @@ def main():
     ws.set_groups(list(groups.iter(state.scan, 999, True)))
     kt = time_kernels(ws, groups.iter, torch, G)
     modeled = ('forward_slice', 'lsqml_backward', 'lsqml_backward_slice0', 'shift_probes', 'farfield',
-               'lsqml_object_update_slices', 'group_forward', 'lsqml_backward_sweep')
+               'lsqml_object_update_slices', 'lsqml_object_update_slices_begin', 'group_forward',
+               'lsqml_backward_sweep')
     dom = max((k for k in kt if k in modeled), key=lambda k: kt[k]['total_ms'])
     NN, M, S, HW = ws.NN, ws.M, ws.S, ws.H * ws.W
     alg = kernel_bytes(dom, B, M, S, NN, HW)
