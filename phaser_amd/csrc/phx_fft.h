@@ -52,13 +52,42 @@ __device__ __forceinline__ float2 w32(int k) {
   return make_float2(c[k], s[k]);
 }
 
-// v * exp(-2 pi i k / 32) (k compile-time after unrolling; trivial factors special-cased)
-__device__ __forceinline__ float2 twmul32(float2 v, int k) {
-  if (k == 0) return v;
-  if (k == 8) return make_float2(v.y, -v.x);
-  if (k == 16) return make_float2(-v.x, -v.y);
-  if (k == 24) return make_float2(-v.y, v.x);
-  return cmul(v, w32(k));
+// Radix-2 butterfly a = u + W v, b = u - W v with W = W_32^k (k compile-time after
+// unrolling).  Trivial factors need no multiply.  The others use the tangent form of the
+// rotation: W v = g p with p = v + i (h/g) v-type terms (g the larger of |Re W|, |Im W|),
+// so the product and the add/subtract take 2 + 4 FMAs instead of a complex multiply
+// (4) plus an add and a subtract (4).
+__device__ __forceinline__ void bfly32(float2& a, float2& b, float2 u, float2 v, int k) {
+  if (k == 0) {
+    a = cadd(u, v);
+    b = csub(u, v);
+    return;
+  }
+  if (k == 8 || k == 24) {  // W = -i or +i
+    const float2 t = k == 8 ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
+    a = cadd(u, t);
+    b = csub(u, t);
+    return;
+  }
+  if (k == 16) {
+    a = csub(u, v);
+    b = cadd(u, v);
+    return;
+  }
+  const float2 w = w32(k);
+  float g;
+  float2 p;
+  if (fabsf(w.x) >= fabsf(w.y)) {  // W v = w.x (v.x - r v.y, v.y + r v.x), r = w.y / w.x
+    const float r = w.y / w.x;
+    g = w.x;
+    p = make_float2(fmaf(-r, v.y, v.x), fmaf(r, v.x, v.y));
+  } else {  // W v = w.y (r v.x - v.y, v.x + r v.y), r = w.x / w.y
+    const float r = w.x / w.y;
+    g = w.y;
+    p = make_float2(fmaf(r, v.x, -v.y), fmaf(r, v.y, v.x));
+  }
+  a = make_float2(fmaf(g, p.x, u.x), fmaf(g, p.y, u.y));
+  b = make_float2(fmaf(-g, p.x, u.x), fmaf(-g, p.y, u.y));
 }
 
 template <int R> struct Log2;
@@ -158,10 +187,7 @@ __device__ __forceinline__ void dft_pow2(float2 (&x)[R]) {
       for (int j = 0; j < half; ++j) {
         int k = j * (32 / len);
         if (INV) k = (32 - k) & 31;
-        const float2 v = twmul32(x[i + j + half], k);
-        const float2 u = x[i + j];
-        x[i + j] = cadd(u, v);
-        x[i + j + half] = csub(u, v);
+        bfly32(x[i + j], x[i + j + half], x[i + j], x[i + j + half], k);
       }
     }
   }

@@ -196,3 +196,34 @@ def test_fft256_split_plane_scheme_matches_phaser_convention():
     a = rng.normal(size=(256, 256)) + 1j * rng.normal(size=(256, 256))
     numpy.testing.assert_allclose(_fft2_256(a), onum.fft2(a), atol=1e-10)
     numpy.testing.assert_allclose(_fft2_256(a, True), onum.ifft2(a), atol=1e-10)
+
+
+def _bfly32(u, v, k):
+    """phx_fft.h bfly32 in float32: a = u + W v, b = u - W v, W = W_32^k, with the
+    rotation in tangent form (g the larger of |Re W|, |Im W|, r the other over g)."""
+    f = numpy.float32
+    w = numpy.complex64(numpy.exp(-2j * numpy.pi * k / 32))
+    if k in (0, 8, 16, 24):
+        t = v * w
+        return u + t, u - t
+    wx, wy = f(w.real), f(w.imag)
+    if abs(wx) >= abs(wy):
+        r, g = f(wy / wx), wx
+        p = (f(v.real - r * v.imag), f(v.imag + r * v.real))
+    else:
+        r, g = f(wx / wy), wy
+        p = (f(r * v.real - v.imag), f(v.real + r * v.imag))
+    a = complex(f(u.real + g * p[0]), f(u.imag + g * p[1]))
+    b = complex(f(u.real - g * p[0]), f(u.imag - g * p[1]))
+    return a, b
+
+
+def test_tangent_form_butterflies_match_complex_product():
+    rng = numpy.random.default_rng(32)
+    for k in range(32):
+        for _ in range(20):
+            u, v = (complex(*rng.normal(size=2)) for _ in range(2))
+            w = numpy.exp(-2j * numpy.pi * k / 32)
+            a, b = _bfly32(numpy.complex64(u), numpy.complex64(v), k)
+            assert abs(a - (u + w * v)) < 1e-5 * (abs(u) + abs(v) + 1)
+            assert abs(b - (u - w * v)) < 1e-5 * (abs(u) + abs(v) + 1)
