@@ -207,6 +207,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--workload", choices=("config2", "config5"), default="config2",
+                    help="config5: the same LSQ-ML measurement at 24 slices, 192x192 x 8 modes, 36864 "
+                         "positions (no CPU baseline, no time-to-NRMSE leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-groups", type=int, default=8)
     ap.add_argument("--no-learned", action="store_true",
@@ -236,16 +239,24 @@ def main():
     from phaser_amd.groups import GroupManager
     from phaser_amd.optics import make_propagators
     from phaser_amd.plan import ConventionalEnginePlan
-    from phaser_amd.synthetic import CONFIG_2, make_dataset
+    from phaser_amd.synthetic import CONFIG_2, CONFIG_5, make_dataset
 
-    cfg = CONFIG_2
+    big = args.workload == "config5"
+    cfg = CONFIG_5 if big else CONFIG_2
+    if big:
+        args.no_cpu = args.no_learned = True
     log("generating synthetic data (GPU forward model + Poisson noise)")
     t0 = time.perf_counter()
     state0, pats, truth = make_dataset(cfg)
     log(f"data ready in {time.perf_counter() - t0:.1f} s: patterns {pats.patterns.shape}, object {state0.object.data.shape}")
 
-    plan = ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=args.warmup + args.steps,
-                                  grouping=64)
+    # at 24 slices from a flat start, full LSQ-ML steps (beta 1, and 0.5) overflow the
+    # object-magnitude product within the first groups, in the oracle as on the GPU; the
+    # config-5 measurement steps with beta 0.05 (the same work per iteration)
+    from phaser_amd.plan import SOLVERS
+    beta = 0.05 if big else 1.0
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver=SOLVERS['lsqml'](beta_object=beta, beta_probe=beta),
+                                  niter=args.warmup + args.steps, grouping=64)
     state = state0.to_device()
     groups = GroupManager(state.scan, plan.grouping, False, seed=0)
     props = make_propagators(state.probe.sampling, state.wavelength, state.object.thicknesses, plan.bwlim_frac)
@@ -300,7 +311,7 @@ def main():
     traffic = None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                          "lsqml_multislice_512_b64_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and not big:
         with open(tpath) as f:
             traffic = json.load(f)['kernels'].get(dom, {}).get('traffic_bytes')
     whole = per_pi * value / 1e9
@@ -346,9 +357,10 @@ def main():
         "metric": METRIC, "value": value, "unit": "patterns·iter/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "multislice LSQ-ML: 8 slices of 517x517 (512x512 nominal) complex64, 128x128 probe "
-                               "x 4 modes, 4096 patterns 128x128 f32 (Poisson 1e5), groups of 64, 1 step = 1 "
-                               "iteration over all patterns", "graphs": not args.no_graphs},
+        "config": {"workload": (f"multislice LSQ-ML: {S} slices of {ws.H}x{ws.W} complex64, {ws.n}x{ws.n} probe "
+                                f"x {M} modes, {n_pos} patterns {ws.n}x{ws.n} f32 (Poisson {cfg.dose:.0e}), groups "
+                                "of 64, 1 step = 1 iteration over all patterns"), "graphs": not args.no_graphs,
+                   "beta": beta},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg, "avg_launch_us": kt[dom]['avg_us'],
