@@ -1,6 +1,6 @@
 """Benchmark: multislice LSQ-ML on one MI355X (HIP backend) vs the numpy oracle on the host.
 
-Workload (the 8-slice LSQ-ML configuration): object 8 slices on the 517x517 reconstruction
+Workload (the 8-slice LSQ-ML configuration): object 8 slices on the 518x518 reconstruction
 grid of a 64x64 raster scan (step 6 px, 0.2 A/px, 512x512 nominal), slice thickness 20 A;
 probe 128x128 with 4 modes (300 kV, 20 mrad, 100 A defocus); 4096 patterns of 128x128
 float32 with Poisson noise at 1e5 e-/pattern; amplitude noise model, groups of 64, position
@@ -207,8 +207,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-graphs", action="store_true")
-    ap.add_argument("--workload", choices=("config2", "config5"), default="config2",
-                    help="config5: the same LSQ-ML measurement at 24 slices, 192x192 x 8 modes, 36864 "
+    ap.add_argument("--workload", choices=("128x4-s8", "192x8-s24"), default="128x4-s8",
+                    help="192x8-s24: the same LSQ-ML measurement at 24 slices, 192x192 x 8 modes, 36864 "
                          "positions (no CPU baseline, no time-to-NRMSE leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-groups", type=int, default=8)
@@ -239,10 +239,10 @@ def main():
     from phaser_amd.groups import GroupManager
     from phaser_amd.optics import make_propagators
     from phaser_amd.plan import ConventionalEnginePlan
-    from phaser_amd.synthetic import CONFIG_2, CONFIG_5, make_dataset
+    from phaser_amd.synthetic import CONFIG_2, LSQML_192X8_S24, make_dataset
 
-    big = args.workload == "config5"
-    cfg = CONFIG_5 if big else CONFIG_2
+    big = args.workload == "192x8-s24"
+    cfg = LSQML_192X8_S24 if big else CONFIG_2
     if big:
         args.no_cpu = args.no_learned = True
     log("generating synthetic data (GPU forward model + Poisson noise)")
@@ -252,7 +252,7 @@ def main():
 
     # at 24 slices from a flat start, full LSQ-ML steps (beta 1, and 0.5) overflow the
     # object-magnitude product within the first groups, in the oracle as on the GPU; the
-    # config-5 measurement steps with beta 0.05 (the same work per iteration)
+    # 24-slice measurement steps with beta 0.05 (the same work per iteration)
     from phaser_amd.plan import SOLVERS
     beta = 0.05 if big else 1.0
     plan = ConventionalEnginePlan(noise_model='amplitude', solver=SOLVERS['lsqml'](beta_object=beta, beta_probe=beta),

@@ -42,8 +42,8 @@ class SyntheticConfig:
 CONFIG_1 = SyntheticConfig()
 CONFIG_2 = SyntheticConfig(n=128, n_modes=4, n_slices=8, scan_shape=(64, 64), scan_step_px=6.0, jitter_px=0.0)
 # 24 slices, 192x192 probe x 8 modes, 192 x 192 positions at 7 px, Poisson at 2e4
-CONFIG_5 = SyntheticConfig(n=192, n_modes=8, n_slices=24, scan_shape=(192, 192), scan_step_px=7.0, jitter_px=0.0,
-                           dose=2e4)
+LSQML_192X8_S24 = SyntheticConfig(n=192, n_modes=8, n_slices=24, scan_shape=(192, 192), scan_step_px=7.0,
+                                  jitter_px=0.0, dose=2e4)
 
 
 def raster_scan(cfg: SyntheticConfig):
