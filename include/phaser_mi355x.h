@@ -138,7 +138,8 @@ int phx_group_begin(const phx_group_desc* d, hipStream_t stream);
  * (phaser/engines/common/simulation.py:255). */
 int phx_probe_spectra(const phx_group_desc* d, hipStream_t stream);
 
-/* gom = sum_b |prod_s O_s(cutout b)|^2 (solvers.py:208). */
+/* gom = sum_b |prod_s O_s(cutout b)|^2 (solvers.py:208), summed in a fixed order (no
+ * atomics); gom need not be cleared first. */
 int phx_group_obj_mag(const phx_group_desc* d, hipStream_t stream);
 
 /* psi[0] = ifft2(fft2(probe) * shift_filter(b)) (simulation.py:241-260, optics.py:151-170);
@@ -216,12 +217,15 @@ int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream);
  * needing a whole CU; tmp is an n_modes * n * n complex64 work plane.  n <= 128. */
 int phx_lsqml_probe_apply_strips(const phx_group_desc* d, void* tmp, hipStream_t stream);
 
-/* ePIE backward step at slice i (solvers.py:459-485), maxima first, then the update. */
+/* ePIE backward step at slice i (solvers.py:459-485), maxima first, then the update.  At
+ * slice 0 with update_probe each plane's probe-update spectrum fft2(P_upd) conj(filt_b)
+ * goes to its farfield slot (the far-field step is done with it). */
 int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
                       hipStream_t stream);
 /* O_i += d_obj; d_obj = 0. */
 int phx_epie_object_apply(const phx_group_desc* d, int slice, hipStream_t stream);
-/* P += beta_P ifft2(mean_b d_probe); d_probe = 0. */
+/* d_probe = sum_b farfield[b] in position order (phx_probe_spectrum_reduce), then
+ * P += beta_P ifft2(d_probe / B); d_probe = 0. */
 int phx_epie_probe_apply(const phx_group_desc* d, hipStream_t stream);
 
 /* Dry run rescale factors (solvers.py:131-167): rescale[b] = sum(D*mask)/sum|psi[S-1]|^2
@@ -242,8 +246,9 @@ int phx_position_gradient(const phx_group_desc* d, int dir, hipStream_t stream);
  * params[8]; the group loss (FP64) is added to the group's red.loss slot. */
 int phx_farfield_gradient(const phx_group_desc* d, int poisson, hipStream_t stream);
 /* Adjoint slice step i (reads the gradient from chi_a/chi_b like phx_lsqml_backward):
- * object gradient into grad_obj[i]; for i > 0 back-propagates; for i == 0 accumulates
- * the probe-gradient spectrum in d_probe and the position gradients in red.pos_num. */
+ * object gradient into grad_obj[i]; for i > 0 back-propagates; for i == 0 writes each
+ * plane's probe-gradient spectrum to its farfield slot (for phx_gradient_probe_finish)
+ * and accumulates the position gradients in red.pos_num. */
 int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, int want_probe, int want_positions,
                           hipStream_t stream);
 /* The far-field gradient update alone (after phx_group_forward): chi_a = dL/d(exit wave)
@@ -253,8 +258,9 @@ int phx_farfield_gradient_update(const phx_group_desc* d, int poisson, hipStream
  * forward model): from the exit-wave gradient in chi_a, every slice in one launch with
  * the gradient carried in registers: with want_object the wave gradient g at slice i is
  * stored as chi_i (chi_slices[i], chi_a at i = S-1) for phx_gradient_object_gather,
- * g <- g conj(O_i), g <- ifft2(fft2(g) conj(prop_{i-1})); at slice 0 the probe-gradient
- * spectrum (d_probe) and the position gradients (red pos_num).  Needs every psi_i
+ * g <- g conj(O_i), g <- ifft2(fft2(g) conj(prop_{i-1})); at slice 0 each plane's
+ * probe-gradient spectrum (into its farfield slot, summed by phx_gradient_probe_finish)
+ * and the position gradients (red pos_num).  Needs every psi_i
  * stored (not the recompute mode) and, for the object gradient with S > 1, chi_slices. */
 int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe, int want_positions,
                        hipStream_t stream);
@@ -275,7 +281,9 @@ int phx_gradient_object_gather(const phx_group_desc* d, int first, int count, hi
  * differ from d->red; the gather reads none of next's buffers. */
 int phx_gradient_object_gather_begin(const phx_group_desc* d, int first, int count, const phx_group_desc* next,
                                      hipStream_t stream);
-/* grad_probe = ifft2(d_probe) (ortho); d_probe = 0. */
+/* d_probe = sum_b farfield[b] in position order (the per-position spectra of the adjoint's
+ * slice 0; no atomics, so the result does not depend on scheduling), then
+ * grad_probe = ifft2(d_probe) (ortho); d_probe = 0. */
 int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream);
 /* Gradient-engine rescale factors into errors[b]: sum(D) / sum |psi_{S-1} O_{S-1}|^2
  * (gradient/run.py:408-438: last slice included, pattern mask not applied). */

@@ -136,20 +136,37 @@ __global__ void __launch_bounds__(T) probe_spectra_k(phx_group_desc d) {
              [&](int py, int px, float2& v) { Pk[py * N + px] = v; });
 }
 
-// gom += |prod_s O_s|^2 of slot b's cutout (solvers.py:208); grid (n*n/256, bmax), gom
-// zeroed by phx_group_begin.
-__global__ void __launch_bounds__(256) group_obj_mag_k(phx_group_desc d) {
+// gom = sum_b |prod_s O_s|^2 over the group's cutouts (solvers.py:208), in a fixed order:
+// a workgroup owns 64 pixels; its wave w sums the positions b = w, w + GOM_WAVES, .. in
+// increasing b, and the GOM_WAVES partials are added in wave order.  No atomics, so gom
+// (and the probe step that divides by it) does not depend on scheduling.  Grid n*n/64.
+constexpr int GOM_WAVES = 8;
+__global__ void __launch_bounds__(64 * GOM_WAVES) group_obj_mag_k(phx_group_desc d) {
+  __shared__ float part[GOM_WAVES][64];
   const int n = d.n, NN = n * n;
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  const int b = blockIdx.y;
-  if (e >= NN || !slot_active(d, b)) return;
-  const int y = e / n, x = e % n;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  const int gs = *d.group_size;
   const float2* obj = (const float2*)d.object;
   const size_t HW = (size_t)d.obj_h * d.obj_w;
-  const Cut c = cut_of(d, d.group_pos[b]);
-  float2 prod = obj_at(obj, c, y, x);
-  for (int s = 1; s < d.n_slices; ++s) prod = cmul(prod, obj_at(obj + s * HW, c, y, x));
-  atomic_add_f32(&d.gom[e], abs2f(prod));
+  float acc = 0.f;
+  if (e < NN) {
+    const int y = e / n, x = e % n;
+    for (int b = w; b < gs; b += GOM_WAVES) {
+      const Cut c = cut_of(d, d.group_pos[b]);
+      float2 prod = obj_at(obj, c, y, x);
+      for (int s = 1; s < d.n_slices; ++s) prod = cmul(prod, obj_at(obj + s * HW, c, y, x));
+      acc += abs2f(prod);
+    }
+  }
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && e < NN) {
+    float t = part[0][lane];
+#pragma unroll
+    for (int k = 1; k < GOM_WAVES; ++k) t += part[k][lane];
+    d.gom[e] = t;
+  }
 }
 
 // psi0 = ifft2(fft2(P) * filt_b), illumination, norm, optional slice-0 step.
@@ -763,7 +780,10 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   // slice 0: probe-gradient spectrum and position gradients (as gradient_backward_k)
   const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
   const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
-  float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+  // this plane's probe-gradient spectrum goes to its own farfield slot (read only by this
+  // workgroup's far-field pass above); phx_gradient_probe_finish sums the slots over the
+  // group in position order, so the probe gradient does not depend on scheduling
+  float2* Fb = (float2*)d.farfield + ((size_t)b * M + m) * NN;
   float gy = 0.f, gx = 0.f;  // per-lane FP32 partials, FP64 block sums
   pl.forward_regs(x);
 #pragma unroll
@@ -771,7 +791,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     const int py = pl.QL * k + pl.q, px = pl.rc;
     const int e = py * N + px;
     const float2 f = shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]);
-    if (want_probe) atomic_add_c64(&dPk[e], cmulc(x[k], f));
+    if (want_probe) Fb[e] = cmulc(x[k], f);
     if (want_pos) {
       const float2 t = cmulc(cmul(Pk[e], f), x[k]);  // conj(v) * fft2(P) filt
       gy += TWO_PI * d.ky_s[py] * t.y;
@@ -1221,9 +1241,11 @@ __global__ void __launch_bounds__(T) epie_backward_k(phx_group_desc d, int i, in
                 [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); });
   } else if (upd_probe) {
     const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
-    float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+    // per-position spectrum into this plane's farfield slot (the far-field step is done
+    // with it); phx_epie_probe_apply sums the slots in position order
+    float2* Fb = (float2*)d.farfield + ((size_t)b * M + m) * NN;
     pl.forward(src, [&](int py, int px, float2& v) {
-      atomic_add_c64(&dPk[py * N + px], cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px])));
+      Fb[py * N + px] = cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]));
     });
   } else {
     for (int e = threadIdx.x; e < NN; e += T) (void)src(e / N, e % N);
@@ -1328,7 +1350,8 @@ __global__ void __launch_bounds__(T) position_gradient_k(phx_group_desc d, int d
 // differentiated by hand).  chi_in holds g_a = 2 dL/dconj(a_i) for a_i = psi_i * O_i
 // (torch's convention).  Writes g_O += g_a conj(psi_i) into grad_obj[i]; for i > 0 the
 // back-propagated g_{a_{i-1}} = ifft2(fft2(g_a conj(O_i)) conj(prop_{i-1})); for i == 0
-// the probe-gradient spectrum sum_b fft2(g_psi0) conj(filt_b) into d_probe and the
+// the probe-gradient spectrum fft2(g_psi0) conj(filt_b) into the plane's farfield slot
+// (summed over b in order by phx_gradient_probe_finish) and the
 // position gradients dL/dy_b, dL/dx_b = Re sum conj(fft2 g_psi0) fft2(P) filt (-2 pi i k).
 template <int N, int T>
 __global__ void __launch_bounds__(T) gradient_backward_k(phx_group_desc d, int i, int want_obj, int want_probe,
@@ -1362,12 +1385,14 @@ __global__ void __launch_bounds__(T) gradient_backward_k(phx_group_desc d, int i
   } else if (want_probe || want_pos) {
     const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
     const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
-    float2* dPk = (float2*)d.d_probe + (size_t)m * NN;
+    // per-position spectrum into this plane's farfield slot, summed in position order
+    // by phx_gradient_probe_finish (no atomics)
+    float2* Fb = (float2*)d.farfield + ((size_t)b * M + m) * NN;
     float gy = 0.f, gx = 0.f;  // per-lane FP32 partials, FP64 block sums
     pl.forward(src, [&](int py, int px, float2& v) {
       const int e = py * N + px;
       const float2 f = shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]);
-      if (want_probe) atomic_add_c64(&dPk[e], cmulc(v, f));
+      if (want_probe) Fb[e] = cmulc(v, f);
       if (want_pos) {
         const float2 t = cmulc(cmul(Pk[e], f), v);  // conj(v) * fft2(P) filt
         gy += TWO_PI * d.ky_s[py] * t.y;
@@ -1388,7 +1413,8 @@ __global__ void __launch_bounds__(T) gradient_backward_k(phx_group_desc d, int i
   }
 }
 
-// grad_probe[m] = ifft2(d_probe[m]) (natural order, ortho scale); d_probe = 0.
+// grad_probe[m] = ifft2(d_probe[m]) (natural order, ortho scale); d_probe = 0.  d_probe
+// holds the ordered sum of the per-position spectra (probe_spectrum_reduce_k).
 template <int N, int T>
 __global__ void __launch_bounds__(T) gradient_probe_finish_k(phx_group_desc d) {
   PHX_PLANE_SETUP(N, T);
@@ -1570,7 +1596,7 @@ int phx_group_obj_mag(const phx_group_desc* d, hipStream_t stream) {
   if (r) return r;
   if (!d->gom) return PHX_EINVAL;
   const int NN = d->n * d->n;
-  hipLaunchKernelGGL(group_obj_mag_k, dim3((unsigned)((NN + 255) / 256), (unsigned)d->bmax), dim3(256), 0, stream, *d);
+  hipLaunchKernelGGL(group_obj_mag_k, dim3((unsigned)((NN + 63) / 64)), dim3(64 * GOM_WAVES), 0, stream, *d);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }
@@ -1723,7 +1749,7 @@ int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe,
   int r = check_desc(d);
   if (r) return r;
   if (!d->chi_a || (d->n_slices > 1 && !d->props) || (want_object && d->n_slices > 1 && !d->chi_slices) ||
-      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)))
+      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)) || (want_probe && !d->farfield))
     return PHX_EINVAL;
   if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // needs every psi_i
   const bool p0 = want_probe || want_positions;
@@ -1903,7 +1929,7 @@ int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int
   int r = check_desc(d);
   if (r) return r;
   if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || !d->aux || (update_object && !d->d_obj) ||
-      (update_probe && !d->d_probe))
+      (update_probe && (!d->d_probe || !d->farfield)))
     return PHX_EINVAL;
   hipLaunchKernelGGL(epie_maxima_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d, slice);
   PHX_CHECK_LAUNCH();
@@ -1931,7 +1957,11 @@ int phx_epie_object_apply(const phx_group_desc* d, int slice, hipStream_t stream
 int phx_epie_probe_apply(const phx_group_desc* d, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (!d->d_probe || !d->finite_flag) return PHX_EINVAL;
+  if (!d->d_probe || !d->finite_flag || !d->farfield) return PHX_EINVAL;
+  // d_probe = the per-position spectra phx_epie_backward left in the farfield slots at
+  // slice 0, summed in position order
+  r = phx_probe_spectrum_reduce(d, stream);
+  if (r) return r;
 #define L(N, T) hipLaunchKernelGGL((epie_probe_apply_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -1986,7 +2016,7 @@ int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, i
   int r = check_desc(d);
   if (r) return r;
   if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (want_object && !d->grad_obj) ||
-      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)))
+      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)) || (want_probe && !d->farfield))
     return PHX_EINVAL;
 #define L(N, T)                                                                                              \
   hipLaunchKernelGGL((gradient_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, want_object, \
@@ -2000,7 +2030,11 @@ int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, i
 int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (!d->d_probe || !d->grad_probe) return PHX_EINVAL;
+  if (!d->d_probe || !d->grad_probe || !d->farfield) return PHX_EINVAL;
+  // d_probe = the per-position spectra the adjoint left in the farfield slots, summed in
+  // position order
+  r = phx_probe_spectrum_reduce(d, stream);
+  if (r) return r;
 #define L(N, T) \
   hipLaunchKernelGGL((gradient_probe_finish_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);

@@ -154,9 +154,12 @@ def test_n128_multislice_modes(solver):
 def test_band_limit_column_skip_matches(monkeypatch):
     """At 128^2 the band-limited propagators let waves 6-9 of every plane skip their
     column passes (the product is zero there). The fused LSQ-ML sweeps give the same
-    result with and without the skip, to the run-to-run noise of the path: the
-    illumination sums and the FP64 step-size sums are accumulated with atomics, so two
-    runs of one configuration already differ in the last bit."""
+    result with and without the skip: the skipped product is exactly zero.  With two
+    modes every FP32 sum of the LSQ-ML step is ordered (the object-magnitude sum and
+    the probe spectra in position order, the update gathered per tile; adding two
+    modes' intensities commutes), so two runs agree to the last bit and the skip must
+    not change one either (the FP64 step-size sums, added in any order, are read as
+    FP32 scalars)."""
     import phaser_amd.device as dev
     from phaser_amd.engines.conventional import run_engine
     from phaser_amd.plan import ConventionalEnginePlan, SOLVERS
@@ -181,9 +184,10 @@ def test_band_limit_column_skip_matches(monkeypatch):
     def rel(x, y):
         return float(numpy.abs(x - y).max() / numpy.abs(y).max())
     noise = max(rel(a.object.data, a2.object.data), rel(a.probe.data, a2.probe.data))
-    assert noise < 1e-5
-    assert rel(a.object.data, b.object.data) < 1e-5
-    assert rel(a.probe.data, b.probe.data) < 1e-5
+    e = max(rel(a.object.data, b.object.data), rel(a.probe.data, b.probe.data))
+    print(f"run-to-run {noise:.2e}, skip vs no skip {e:.2e}", flush=True)
+    assert noise <= 1e-7
+    assert e <= 1e-7
 
 
 def test_lsqml_deep_stack_block_sums():
@@ -198,8 +202,10 @@ def test_lsqml_deep_stack_block_sums():
 
 def test_probe_step_strips_match_single_launch(monkeypatch):
     """The probe step in three strip launches (beside the object update) against the
-    one-workgroup-per-mode launch after the reduction, at 128^2 and 64^2 with graphs:
-    equal to the run-to-run noise of the path (atomic sums)."""
+    one-workgroup-per-mode launch after the reduction, at 128^2 and 64^2 with graphs.
+    With two modes the path is deterministic (ordered sums), so two runs must agree to
+    the last bit; the two probe-step forms run the same transforms in different launch
+    shapes and must agree to FP32 rounding."""
     import phaser_amd.device as dev
     from phaser_amd.engines.conventional import run_engine
     from phaser_amd.plan import ConventionalEnginePlan, SOLVERS
@@ -219,10 +225,10 @@ def test_probe_step_strips_match_single_launch(monkeypatch):
         def rel(x, y):
             return float(numpy.abs(x - y).max() / numpy.abs(y).max())
         noise = max(rel(a.object.data, a2.object.data), rel(a.probe.data, a2.probe.data))
-        print(n, 'noise', noise, 'strips vs single', rel(a.probe.data, b.probe.data), flush=True)
-        assert noise < 1e-5
-        assert rel(a.object.data, b.object.data) < 1e-5
-        assert rel(a.probe.data, b.probe.data) < 1e-5
+        e = max(rel(a.object.data, b.object.data), rel(a.probe.data, b.probe.data))
+        print(n, f'run-to-run {noise:.2e}, strips vs single {e:.2e}', flush=True)
+        assert noise <= 1e-7
+        assert e <= 1e-6
 
 
 def test_device_resident_patterns():
@@ -319,7 +325,9 @@ def test_update_launch_clearing_next_group_matches_separate_clear(monkeypatch):
     """The object update that also zeroes the next group's accumulators
     (phx_lsqml_object_update_slices_begin) against phx_group_begin launched on its own
     before each group, with graphs (probe update switching on after the first
-    iteration) and without: equal to the run-to-run noise of the path (atomic sums)."""
+    iteration) and without.  The fold only moves a clear into another launch, and with
+    two modes the path is deterministic (ordered sums), so both must agree to the last
+    bit."""
     import phaser_amd.device as dev
     from phaser_amd.engines.conventional import run_engine
     from phaser_amd.plan import ConventionalEnginePlan, SOLVERS
@@ -338,9 +346,8 @@ def test_update_launch_clearing_next_group_matches_separate_clear(monkeypatch):
         monkeypatch.setattr(dev, 'FOLD_GROUP_BEGIN', False)
         b = run_engine(dict(args, state=state.copy()), plan, use_graphs=graphs).to_numpy()
         noise = max(rel(a.object.data, a2.object.data), rel(a.probe.data, a2.probe.data))
-        print('graphs', graphs, 'noise', noise, 'fold vs separate', rel(a.object.data, b.object.data),
-              rel(a.probe.data, b.probe.data), flush=True)
-        assert noise < 1e-5
-        assert rel(a.object.data, b.object.data) < 1e-5
-        assert rel(a.probe.data, b.probe.data) < 1e-5
-        numpy.testing.assert_allclose(a.progress.detector_errors, b.progress.detector_errors, rtol=1e-5)
+        e = max(rel(a.object.data, b.object.data), rel(a.probe.data, b.probe.data))
+        print('graphs', graphs, f'run-to-run {noise:.2e}, fold vs separate {e:.2e}', flush=True)
+        assert noise <= 1e-7
+        assert e <= 1e-7
+        numpy.testing.assert_allclose(a.progress.detector_errors, b.progress.detector_errors, rtol=1e-6)

@@ -214,8 +214,11 @@ def test_gradient_engine_matches_oracle(solver):
 def test_gather_clearing_next_group_matches_separate_clear(monkeypatch):
     """The object-gradient gather that also zeroes the next group's accumulators
     (phx_gradient_object_gather_begin) against phx_group_begin before every group (SGD
-    with momentum, 3 groups, 3 iterations): equal to the run-to-run noise of the path
-    (the probe-gradient spectrum is an atomic sum over positions)."""
+    with momentum, 3 groups, 3 iterations).  Every sum that feeds the object and the
+    probe is ordered (object-gradient gather, per-position probe spectra summed in
+    position order, the two modes' intensities), so two runs agree to the last bit and
+    the fold must not change a bit either; the loss is an FP64 sum of per-position
+    losses added in any order, read as FP32."""
     import phaser_amd.engines.gradient as gr
     from phaser_amd.plan import GradientEnginePlan, SGDSolverPlan
     _cfg, state, pats = _problem(S=2, M=2, scan=(6, 6), seed=23)
@@ -232,6 +235,7 @@ def test_gather_clearing_next_group_matches_separate_clear(monkeypatch):
     for name, f in (('object', lambda s: s.object.data), ('probe', lambda s: s.probe.data),
                     ('loss', lambda s: s.progress.detector_errors)):
         noise, e = rel_err(f(a), f(a2)), rel_err(f(a), f(b))
-        print(f"{name}: noise {noise:.2e}, fold vs separate {e:.2e}", flush=True)
-        assert noise < 1e-5
-        assert e < 1e-5
+        print(f"{name}: run-to-run {noise:.2e}, fold vs separate {e:.2e}", flush=True)
+        bar = 1e-6 if name == 'loss' else 1e-7
+        assert noise <= bar
+        assert e <= bar
