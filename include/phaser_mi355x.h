@@ -2,7 +2,8 @@
  *
  * Every entry point takes device pointers, plain sizes and a hipStream_t, launches its
  * kernel(s) on that stream and returns 0 on success, PHX_EINVAL (-1000) for a bad
- * argument, PHX_EUNSUPPORTED (-1001) for a size without a kernel, or a negative
+ * argument, PHX_EUNSUPPORTED (-1001) for a size without a kernel, PHX_ETOOLARGE (-1002)
+ * for a size beyond a fixed-size device array of the library, or a negative
  * hipError_t.  Nothing allocates, nothing synchronises, no C++ exception crosses the ABI.
  * Complex arrays are interleaved complex64 (float re, float im), C-contiguous.
  *
@@ -216,6 +217,18 @@ int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream);
  * lines of the plane) that run beside a concurrent object update instead of each
  * needing a whole CU; tmp is an n_modes * n * n complex64 work plane.  n <= 128. */
 int phx_lsqml_probe_apply_strips(const phx_group_desc* d, void* tmp, hipStream_t stream);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * probe step of phx_lsqml_probe_apply_strips (same kernels, same values) with its work
+ * plane in a fixed device array of the library instead of a `tmp` argument.  It calls no
+ * hipMalloc/hipFree.  Limit: n <= 128 and n_modes * n * n <= phx_probe_strip_scratch_elems()
+ * (131072 complex64, 1 MiB), else PHX_ETOOLARGE.  Every call on one GPU uses that GPU's
+ * single array, so calls that overlap on one GPU (other streams, other threads) race;
+ * serialise them.  The array's device address is looked up on the first call on each
+ * device (that device must be current). */
+int phx_lsqml_probe_apply_strips_static(const phx_group_desc* d, hipStream_t stream);
+/* Capacity, in complex64 elements, of phx_lsqml_probe_apply_strips_static's work array. */
+int64_t phx_probe_strip_scratch_elems(void);
 
 /* ePIE backward step at slice i (solvers.py:459-485), maxima first, then the update.  At
  * slice 0 with update_probe each plane's probe-update spectrum fft2(P_upd) conj(filt_b)

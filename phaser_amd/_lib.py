@@ -17,6 +17,7 @@ c_vp = ctypes.c_void_p
 
 PHX_EINVAL = -1000
 PHX_EUNSUPPORTED = -1001
+PHX_ETOOLARGE = -1002
 
 
 class GroupDesc(ctypes.Structure):
@@ -84,6 +85,8 @@ SIGNATURES = {
     "phx_probe_spectrum_reduce": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply": (c_int, [_P, c_vp]),
     "phx_lsqml_probe_apply_strips": (c_int, [_P, c_vp, c_vp]),
+    "phx_lsqml_probe_apply_strips_static": (c_int, [_P, c_vp]),
+    "phx_probe_strip_scratch_elems": (c_i64, []),
     "phx_epie_backward": (c_int, [_P, c_int, c_int, c_int, c_vp]),
     "phx_epie_object_apply": (c_int, [_P, c_int, c_vp]),
     "phx_epie_probe_apply": (c_int, [_P, c_vp]),
@@ -137,6 +140,8 @@ def check(rc: int, what: str):
         raise ValueError(f"{what}: invalid argument")
     if rc == PHX_EUNSUPPORTED:
         raise NotImplementedError(f"{what}: size not supported by the HIP kernels")
+    if rc == PHX_ETOOLARGE:
+        raise ValueError(f"{what}: size exceeds the library's fixed device work array")
     raise HipError(f"{what}: HIP error {-rc}")
 
 

@@ -6,6 +6,7 @@
 #define PHX_OK 0
 #define PHX_EINVAL (-1000)     // bad argument (shape, size, null pointer)
 #define PHX_EUNSUPPORTED (-1001)  // size / combination without a kernel
+#define PHX_ETOOLARGE (-1002)  // larger than a fixed-size device buffer of the library
 
 // Return a negative hipError_t on launch failure.
 #define PHX_CHECK_LAUNCH()                                   \

@@ -12,6 +12,7 @@
 #include "../../include/phaser_mi355x.h"
 
 #include <climits>
+#include <mutex>
 
 using namespace phx;
 
@@ -1562,6 +1563,39 @@ static void launch_lsqml_backward(const phx_group_desc& d, int slice, int upd_ob
 #undef K
 }
 
+// ---- work plane of the no-allocation probe step (phx_lsqml_probe_apply_strips_static) ----
+// A file-scope device array: every GPU the library runs on holds one copy at its own
+// address until the process ends.  Host code never passes the array's name to a kernel
+// (in host code that name is a stand-in, not a device address); it uses the address
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+constexpr int64_t PROBE_STRIP_SCRATCH_C64 = 131072;  // 1 MiB: n_modes * n * n <= 2^17
+static __device__ __align__(256) float2 probe_strip_scratch[PROBE_STRIP_SCRATCH_C64];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::once_flag g_scratch_once[SCRATCH_MAX_DEVICES];
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES];
+hipError_t g_scratch_err[SCRATCH_MAX_DEVICES];
+
+// Device address of probe_strip_scratch on the current device (looked up on the first call
+// made on that device, with that device current).  0 or a negative hipError_t.
+int probe_strip_scratch_ptr(float2** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return -(int)e;
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return PHX_EINVAL;
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(probe_strip_scratch));
+    g_scratch_ptr[dev] = p;
+  });
+  if (g_scratch_err[dev] != hipSuccess) return -(int)g_scratch_err[dev];
+  if (!g_scratch_ptr[dev]) return PHX_EINVAL;
+  *out = (float2*)g_scratch_ptr[dev];
+  return PHX_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int64_t phx_red_doubles(int32_t n_slices, int32_t bmax, int32_t n_modes) {
@@ -1922,6 +1956,21 @@ int phx_lsqml_probe_apply_strips(const phx_group_desc* d, void* tmp, hipStream_t
 #undef L
   PHX_CHECK_LAUNCH();
   return PHX_OK;
+}
+
+int64_t phx_probe_strip_scratch_elems(void) { return PROBE_STRIP_SCRATCH_C64; }
+
+int phx_lsqml_probe_apply_strips_static(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (d->n > 128) return PHX_EUNSUPPORTED;
+  if ((int64_t)d->n_modes * d->n * d->n > PROBE_STRIP_SCRATCH_C64) return PHX_ETOOLARGE;
+  float2* t = nullptr;
+  r = probe_strip_scratch_ptr(&t);
+  if (r) return r;
+  // every element of the work plane is written (inverse column pass) before it is read,
+  // so nothing left by an earlier call needs clearing
+  return phx_lsqml_probe_apply_strips(d, t, stream);
 }
 
 int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
