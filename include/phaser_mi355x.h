@@ -63,6 +63,12 @@ int phx_fft2_c64(const void* in, void* out, int64_t batch, int64_t n, int invers
 int phx_scramble(const void* in, void* out, int64_t batch, int64_t n, int is_complex, int unscramble,
                  hipStream_t stream);
 
+/* check_finite (phaser/utils/num.py:535-541) as a device flag: *flag = min(*flag, value)
+ * when any of `count` floats at ptr is NaN or inf.  The engines run it over the object
+ * and probe before an iteration's first group, so state that is already non-finite is
+ * reported as group 0, where solvers.py:109 raises for it. */
+int phx_check_finite_f32(const void* ptr, int64_t count, int32_t* flag, int32_t value, hipStream_t stream);
+
 /* ptr[i] *= scale for `count` complex64 values (presolve probe rescale, solvers.py:60). */
 int phx_scale_c64(void* ptr, int64_t count, float scale, hipStream_t stream);
 
@@ -305,6 +311,43 @@ int phx_dry_run_stats_gradient(const phx_group_desc* d, hipStream_t stream);
  * sum_m |F_m|^2 / n^2 = sum_m |psi_{S-1} O_{S-1}|^2 by Parseval): the dry run as one
  * forward launch and this reduction. */
 int phx_dry_run_stats_intensity(const phx_group_desc* d, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * Constraints and cost regularizers (phaser/engines/common/regularizers.py), applied
+ * after each group (group constraints: solvers.py:113, gradient/run.py:344-347), after
+ * each iteration (iter constraints: conventional/run.py:93, gradient/run.py:276-279) or in
+ * the gradient engine's loss (regularizers: gradient/run.py:399-403).
+ * --------------------------------------------------------------------------------- */
+/* ClampObjectAmplitude (regularizers.py:20-42): o *= min(|o|, amplitude) / |o| over
+ * `count` complex64 values (|o| = 0 gives NaN, as numpy).  With a flag, a non-finite
+ * result records flag_value (min). */
+int phx_clamp_amplitude_c64(void* obj, int64_t count, float amplitude, int32_t* flag, int32_t flag_value,
+                            hipStream_t stream);
+/* LimitProbeSupport (regularizers.py:45-64): every probe mode P <- ifft2(fft2(P) mask) with
+ * mask (n*n, 0/1, scrambled order) the k-disk of max_angle; probe_k = fft2 of the result. */
+int phx_probe_limit_support(const phx_group_desc* d, const float* mask_s, hipStream_t stream);
+/* RegularizeLayers (regularizers.py:86-132): per pixel, the complex log of the n_slices
+ * slices convolved along the slice axis with taps[0 .. 2 radius] (edge slices repeated,
+ * scipy.ndimage.convolve1d mode 'nearest'; FP64 sums), exponentiated and blended:
+ * o <- weight exp(.) + one_minus_weight o.  n_slices <= 128. */
+int phx_regularize_layers_c64(void* obj, int64_t n_slices, int64_t obj_h, int64_t obj_w, const float* taps,
+                              int64_t radius, float weight, float one_minus_weight, hipStream_t stream);
+/* Object cost regularizers of the gradient engine (regularizers.py:195-226 obj_l1/obj_l2,
+ * :297-316 obj_tikh, :319-338 layers_tv, :341-360 layers_tikh): coefs[5] are their costs
+ * times the group's cost scale (B / n_positions), in that order (0 = off).  Adds their
+ * loss to *loss (FP64) and, with want_grad, their gradient 2 dL/dconj(O) to grad. */
+int phx_cost_regularizers_c64(const void* obj, void* grad, int64_t n_slices, int64_t obj_h, int64_t obj_w,
+                              const float* coefs, double* loss, int want_grad, hipStream_t stream);
+/* ObjLowPass / ObjGaussian (regularizers.py:135-192): every slice O <- ifft2(F fft2(O)) for
+ * a real H x W filter F on the corner-origin frequency grid, given transposed (filt_t[kx][ky]).
+ * Object sides are arbitrary: each 1-D DFT is Bluestein's, a circular convolution of
+ * length L = 2^p >= 2N - 1 (L <= 4096: sides up to 2048) with host tables per side: chirp (N) exp(-i pi k^2/N),
+ * bhat (L) FFT_L(conj chirp)/L, tw (L/2) exp(-2 pi i j/L).  work: n_slices*obj_h*obj_w
+ * complex64 (the transposed object). */
+int phx_object_filter_c64(void* obj, void* work, int64_t n_slices, int64_t obj_h, int64_t obj_w,
+                          const float* filt_t, const void* chirp_h, const void* bhat_h, const void* tw_h,
+                          int64_t l_h, const void* chirp_w, const void* bhat_w, const void* tw_w, int64_t l_w,
+                          hipStream_t stream);
 
 /* Optimizer steps on complex64 parameters (gradient/solvers.py:88-138 over optax): the
  * direction is d = -grad / scale; finite_flag (may be NULL) records flag_value on NaN/inf.

@@ -27,7 +27,7 @@ class SteepestDescentPositionSolver:
         self.step_size = step_size
         self.max_step_size = step_size
 
-    def init_state(self, scan):
+    def init_state(self, sim):
         return None
 
     def perform_update(self, positions, gradients, state):
@@ -46,8 +46,8 @@ class MomentumPositionSolver:
         self.max_step_size = max_step_size
         self.momentum = momentum
 
-    def init_state(self, scan):
-        return numpy.zeros_like(scan)
+    def init_state(self, sim):
+        return numpy.zeros_like(sim.scan)
 
     def perform_update(self, positions, gradients, state):
         update = self.step_size * gradients + self.momentum * state
@@ -61,22 +61,28 @@ def run_engine(state, patterns, pattern_mask, *, solver='lsqml', noise_model, ni
                compact=False, shuffle_groups=None, update_probe=True, update_object=True,
                update_positions=False, calc_error=True, calc_error_fraction=0.1, bwlim_frac=2 / 3,
                beta_object=1.0, beta_probe=1.0, illum_reg_object=1e-2, illum_reg_probe=1e-2, gamma=1e-4,
-               position_solver=None, seed=None, dtype=numpy.float32, group_callback=None, max_groups=None):
+               position_solver=None, seed=None, dtype=numpy.float32, group_callback=None, max_groups=None,
+               group_constraints=(), iter_constraints=()):
     """Conventional engine (run.py:15-122) with the LSQ-ML (solvers.py:18-128) or ePIE
     (solvers.py:293-383) solver.  `state` is modified in place and returned.
 
     Solver hyperparameters are constants here (the reference evaluates schedules once
     per iteration, solvers.py:81-85).  `max_groups` truncates every group loop (used to
-    time a bounded CPU sample); `group_callback(i, group)` is called after each group."""
+    time a bounded CPU sample); `group_callback(i, group)` is called after each group.
+    `group_constraints` / `iter_constraints`: oracle.regularizers objects, applied after
+    each group (solvers.py:109-113, after the finite check) and after the group loop
+    (run.py:93, before the position update)."""
     assert patterns.dtype == dtype and pattern_mask.dtype == dtype
     assert state.object.data.dtype == to_complex_dtype(dtype)
     assert state.probe.data.dtype == to_complex_dtype(dtype)
 
     sim = Sim(state, noise_model, dtype)
-    shuffle_flag = shuffle_groups if shuffle_groups is not None else (not compact)
+    shuffle_flag = shuffle_groups or not compact   # run.py:32
     groups = GroupManager(state.scan, grouping, compact, seed=seed)
     calc_error_mask = mask_fraction_of_groups(len(groups), calc_error_fraction)
-    pos_state = None if position_solver is None else position_solver.init_state(state.scan)
+    pos_state = None if position_solver is None else position_solver.init_state(state)
+    gc_states = [c.init_state(state) for c in group_constraints]   # simulation.py:127-132
+    ic_states = [c.init_state(state) for c in iter_constraints]
     start_i = state.iter.total_iter
 
     (ky64, kx64) = state.probe.sampling.recip_grid(dtype=dtype)
@@ -122,12 +128,17 @@ def run_engine(state, patterns, pattern_mask, *, solver='lsqml', noise_model, ni
                     beta_object=beta_object, beta_probe=beta_probe, update_object=upd_obj,
                     update_probe=upd_probe)
             check_finite(state.object.data, state.probe.data, context=f"object or probe, group {gi}")
+            for k, c in enumerate(group_constraints):   # simulation.py:137-146
+                _s, gc_states[k] = c.apply_group(group, state, gc_states[k])
             if iter_update_positions:
                 pos_update[tuple(group)] = gpu
             if group_calc_error:
                 iter_errors.append(errors)
             if group_callback is not None:
                 group_callback(gi, group)
+
+        for k, c in enumerate(iter_constraints):   # run.py:93, simulation.py:148-157
+            _s, ic_states[k] = c.apply_iter(state, ic_states[k])
 
         if iter_update_positions:
             if position_solver is None:

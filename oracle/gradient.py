@@ -13,7 +13,7 @@ jax and optax are absent, so:
 import numpy
 
 from .misc import GroupManager
-from .num import fft2, ifft2, abs2, to_complex_dtype
+from .num import fft2, ifft2, abs2, check_finite, to_complex_dtype
 from .optics import fourier_shift_filter, make_propagators
 
 
@@ -143,25 +143,37 @@ class Solver:
 
 
 def run_engine(state, patterns, mask, *, noise, solvers, niter, grouping=None, seed=None, bwlim_frac=2 / 3,
-               dtype=numpy.float32, update_object=True, update_probe=True, update_positions=False):
+               dtype=numpy.float32, update_object=True, update_probe=True, update_positions=False,
+               compact=False, shuffle_groups=None, regularizers=(), group_constraints=(), iter_constraints=()):
     """Gradient engine loop (run.py:156-290) with group solvers for object/probe and
-    per-iteration solvers for positions.  `solvers`: list of oracle Solver."""
+    per-iteration solvers for positions.  `solvers`: list of oracle Solver.
+    `regularizers`: oracle.regularizers cost regularizers, whose loss joins the group loss
+    and whose object gradient joins the model's (run.py:399-403); constraints are applied
+    after each group's solver steps (run.py:344-347) and after the per-iteration solvers
+    (run.py:276-279)."""
     cdt = to_complex_dtype(dtype)
     ky, kx = state.probe.sampling.recip_grid(dtype=dtype)
     props = make_propagators(ky, kx, state.probe.sampling.k_max, state.wavelength, state.object.thicknesses, bwlim_frac)
-    groups = GroupManager(state.scan, grouping, False, seed)
+    groups = GroupManager(state.scan, grouping, compact, seed)
+    shuffle = shuffle_groups or not compact     # run.py:185
     factors = numpy.concatenate([dry_run(state, g, patterns, ky, kx, props) for g in groups.iter(state.scan)])
     state.probe.data *= numpy.sqrt(numpy.mean(factors))
     probe_int = numpy.sum(abs2(state.probe.data))
     group_solvers = [s for s in solvers if not (s.params <= {'positions', 'tilt'})]
     iter_solvers = [s for s in solvers if s.params <= {'positions', 'tilt'}]
     start_i = state.iter.total_iter
+    gc_states = [c.init_state(state) for c in group_constraints]
+    ic_states = [c.init_state(state) for c in iter_constraints]
     for i in range(1, niter + 1):
         state.iter.engine_iter = i
         losses = []
         pos_grads = numpy.zeros_like(state.scan)
-        for group in groups.iter(state.scan, i, True):
+        for gi, group in enumerate(groups.iter(state.scan, i, shuffle)):
             loss, g_obj, g_probe, g_pos = model_grad(state, group, patterns, mask, noise, ky, kx, props)
+            for reg in regularizers:
+                rl, _st = reg.calc_loss_group(group, state, None)
+                loss = loss + rl
+                g_obj = g_obj + reg.grad_group(group, state)
             B = group.shape[-1]
             dirs = {'object': (-g_obj / (probe_int * B)).astype(cdt), 'probe': (-g_probe / B).astype(cdt)}
             pos_grads[tuple(group)] = g_pos / (probe_int * B)
@@ -174,13 +186,18 @@ def run_engine(state, patterns, mask, *, noise, solvers, niter, grouping=None, s
                         state.object.data += upd['object'].astype(cdt)
                     if 'probe' in upd:
                         state.probe.data += upd['probe'].astype(cdt)
+            for k, c in enumerate(group_constraints):
+                _s, gc_states[k] = c.apply_group(group, state, gc_states[k])
             losses.append(loss)
+            check_finite(state.object.data, state.probe.data, context=f"object or probe, group {gi}")
         loss = float(numpy.mean(numpy.array(losses, dtype=dtype)))
         if update_positions:
             for s in iter_solvers:
                 upd = s.update({'positions': -pos_grads}, loss)['positions']
                 upd = upd - numpy.mean(upd, tuple(range(upd.ndim - 1)))
                 state.scan += upd.astype(state.scan.dtype)
+        for k, c in enumerate(iter_constraints):
+            _s, ic_states[k] = c.apply_iter(state, ic_states[k])
         state.progress.iters = numpy.concatenate([state.progress.iters, [i + start_i]])
         state.progress.detector_errors = numpy.concatenate([state.progress.detector_errors, [loss]])
     return state

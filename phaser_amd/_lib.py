@@ -51,6 +51,13 @@ SIGNATURES = {
     "phx_fft2_c64": (c_int, [c_vp, c_vp, c_i64, c_i64, c_int, c_vp, c_vp]),
     "phx_scramble": (c_int, [c_vp, c_vp, c_i64, c_i64, c_int, c_int, c_vp]),
     "phx_scale_c64": (c_int, [c_vp, c_i64, c_f32, c_vp]),
+    "phx_check_finite_f32": (c_int, [c_vp, c_i64, c_vp, c_i32, c_vp]),
+    "phx_clamp_amplitude_c64": (c_int, [c_vp, c_i64, c_f32, c_vp, c_i32, c_vp]),
+    "phx_probe_limit_support": (c_int, [_P, c_vp, c_vp]),
+    "phx_regularize_layers_c64": (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_f32, c_f32, c_vp]),
+    "phx_cost_regularizers_c64": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_int, c_vp]),
+    "phx_object_filter_c64": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp,
+                                      c_vp, c_i64, c_vp]),
     "phx_red_doubles": (c_i64, [c_i32, c_i32, c_i32]),
     "phx_group_begin": (c_int, [_P, c_vp]),
     "phx_probe_spectra": (c_int, [_P, c_vp]),

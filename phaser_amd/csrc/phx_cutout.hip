@@ -134,6 +134,14 @@ int launch_scramble(const void* in, void* out, int64_t batch, int is_complex, in
   return PHX_OK;
 }
 
+// flag = min(flag, value) if any of the n floats is NaN or inf (one atomic per wave).
+__global__ void check_finite_k(const float* p, int64_t n, int32_t* flag, int32_t value) {
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    bad |= !isfinite(p[i]);
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicMin(flag, value);
+}
+
 __global__ void scale_c64_k(float2* p, int64_t n, float s) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = cscale(p[i], s);
@@ -332,6 +340,17 @@ int phx_scramble(const void* in, void* out, int64_t batch, int64_t n, int is_com
     case 192: return launch_scramble<192>(in, out, batch, is_complex, unscramble, stream);
     default: return PHX_EUNSUPPORTED;
   }
+}
+
+int phx_check_finite_f32(const void* ptr, int64_t count, int32_t* flag, int32_t value, hipStream_t stream) {
+  if (!ptr || !flag || count < 0) return PHX_EINVAL;
+  if (count == 0) return PHX_OK;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(check_finite_k, dim3((unsigned)blocks), dim3(256), 0, stream, (const float*)ptr, count, flag,
+                     value);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
 }
 
 int phx_scale_c64(void* ptr, int64_t count, float scale, hipStream_t stream) {

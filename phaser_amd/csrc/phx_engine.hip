@@ -150,14 +150,27 @@ __global__ void __launch_bounds__(64 * GOM_WAVES) group_obj_mag_k(phx_group_desc
   const int gs = *d.group_size;
   const float2* obj = (const float2*)d.object;
   const size_t HW = (size_t)d.obj_h * d.obj_w;
+  constexpr int K = 4;  // positions per pass, their loads in flight together
   float acc = 0.f;
   if (e < NN) {
     const int y = e / n, x = e % n;
-    for (int b = w; b < gs; b += GOM_WAVES) {
-      const Cut c = cut_of(d, d.group_pos[b]);
-      float2 prod = obj_at(obj, c, y, x);
-      for (int s = 1; s < d.n_slices; ++s) prod = cmul(prod, obj_at(obj + s * HW, c, y, x));
-      acc += abs2f(prod);
+    for (int b0 = w; b0 < gs; b0 += K * GOM_WAVES) {
+      Cut c[K];
+      float2 prod[K];
+#pragma unroll
+      for (int u = 0; u < K; ++u) {
+        const int b = b0 + u * GOM_WAVES;
+        c[u] = cut_of(d, d.group_pos[b < gs ? b : w]);
+        prod[u] = obj_at(obj, c[u], y, x);
+      }
+      for (int s = 1; s < d.n_slices; ++s) {
+#pragma unroll
+        for (int u = 0; u < K; ++u) prod[u] = cmul(prod[u], obj_at(obj + s * HW, c[u], y, x));
+      }
+      // positions b0, b0 + GOM_WAVES, .. in increasing order
+#pragma unroll
+      for (int u = 0; u < K; ++u)
+        if (b0 + u * GOM_WAVES < gs) acc += abs2f(prod[u]);
     }
   }
   part[w][lane] = acc;
@@ -1180,6 +1193,34 @@ __global__ void __launch_bounds__(Strip<N>::T) probe_apply_cols_fwd_k(phx_group_
   for (int k = 0; k < E; ++k) Pk[(QL * k + pl.q) * N + line] = x[k];
 }
 
+// LimitProbeSupport (regularizers.py:45-64): P <- ifft2(fft2(P) mask) per mode, mask the
+// host-computed k-disk |k| <= max_angle / lambda in scrambled order (0 or 1); then
+// probe_k = fft2 of the new probe, as the probe step leaves it, so the next group's
+// forward model starts from current spectra.
+template <int N, int T>
+__global__ void __launch_bounds__(T) probe_support_k(phx_group_desc d, const float* mask_s) {
+  PHX_PLANE_SETUP(N, T);
+  constexpr int NN = N * N;
+  constexpr int E = Plane<N>::E;
+  constexpr float inv_nn = 1.0f / (float)NN;
+  const int m = blockIdx.x;
+  float2* P = (float2*)d.probe + (size_t)m * NN;
+  float2 x[E];
+  pl.load_rows([&](int y, int xx) { return P[y * N + xx]; }, x);
+  pl.convolve_regs(x, [&](int py, int px, float2 v) { return cscale(v, mask_s[py * N + px]); });
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const float2 v = cscale(x[j], inv_nn);
+    P[pl.rc * N + pl.QL * j + pl.q] = v;
+    x[j] = v;
+  }
+  pl.refresh();
+  pl.forward_regs(x);
+  float2* Pk = (float2*)d.probe_k + (size_t)m * NN;
+#pragma unroll
+  for (int k = 0; k < E; ++k) Pk[(pl.QL * k + pl.q) * N + pl.rc] = x[k];
+}
+
 // ---------------------------------- ePIE ------------------------------------------
 // aux[b] = max_px |O_i|^2, aux[bmax + b] = max_px sum_m |psi_i|^2 (solvers.py:462-472)
 __global__ void __launch_bounds__(256) epie_maxima_k(phx_group_desc d, int i) {
@@ -1985,6 +2026,17 @@ int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int
 #define L(N, T)                                                                                           \
   hipLaunchKernelGGL((epie_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, update_object, \
                      update_probe)
+  PHX_DISPATCH(d->n, L);
+#undef L
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_probe_limit_support(const phx_group_desc* d, const float* mask_s, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!mask_s || !d->probe_k) return PHX_EINVAL;
+#define L(N, T) hipLaunchKernelGGL((probe_support_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d, mask_s)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();

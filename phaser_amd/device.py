@@ -253,6 +253,14 @@ class GroupWorkspace:
     def reset_finite_flag(self):
         _lib.call("phx_fill_u32", _ptr(self.finite_flag), INT32_MAX, 1, stream_handle())
 
+    def check_state_finite(self, value=0):
+        """Record `value` in the finite flag if the object or probe already holds NaN/inf
+        (launched before an iteration's first group: the reference's check after that group
+        covers the whole state, solvers.py:109)."""
+        st = stream_handle()
+        for t in (self.object, self.probe):
+            self.call("phx_check_finite_f32", _ptr(t), 2 * t.numel(), _ptr(self.finite_flag), int(value), st)
+
     def finite_group(self):
         v = int(self.finite_flag.item())
         return None if v == INT32_MAX else v
@@ -287,9 +295,20 @@ class GroupWorkspace:
             c("phx_forward_slice", d, i, int(illum), st)
         c("phx_farfield", d, int(illum and self.S > 1), int(calc_error), st)
 
-    def launch_lsqml_group(self, g, *, update_object, update_probe, update_position, calc_error):
-        """lsqml_run (solvers.py:171-290) for group slot g.  Groups are launched in order
-        0, 1, .. within an iteration (the fused path transforms the probe only at g = 0)."""
+    def _launch_constraints(self, constraints, g, main, side):
+        """Group constraints after group g's updates (solvers.py:109-113): object ones on
+        the main stream after the object update, probe ones after the probe step (on the
+        second stream when it ran there).  The reference checks finiteness before applying
+        them, so a NaN they make is reported at the next group (flag g + 1; after the
+        last group, the next iteration's state check reports it)."""
+        flag = g + 1 if g + 1 < self.G else None
+        for con in constraints:
+            con.apply(self, g, flag, main if con.target == 'object' else side)
+
+    def launch_lsqml_group(self, g, *, update_object, update_probe, update_position, calc_error, constraints=()):
+        """lsqml_run (solvers.py:171-290) for group slot g, then its group constraints.
+        Groups are launched in order 0, 1, .. within an iteration (the fused path
+        transforms the probe only at g = 0)."""
         d, st = self.desc_refs[g], stream_handle()
         c = self.call
         if self.fused and (self.chi_slices is not None or self.S == 1):
@@ -364,6 +383,7 @@ class GroupWorkspace:
                     c("phx_lsqml_probe_apply", d, sh)
                 if update_object:
                     c("phx_lsqml_object_update_slices", d, 0, self.S, st)
+            self._launch_constraints(constraints, g, st, sh)
             if side is not None and g == self.G - 1:
                 main.wait_stream(side)
             return
@@ -387,14 +407,16 @@ class GroupWorkspace:
         if update_probe:
             c("phx_probe_spectrum_reduce", d, st)
             c("phx_lsqml_probe_apply", d, st)
+        self._launch_constraints(constraints, g, st, st)
 
     def _launch_position(self, d, st):
         self.call("phx_position_gradient", d, 0, st)
         self.call("phx_position_gradient", d, 1, st)
         self.call("phx_position_finish", d, _ptr(self.pos_update), st)
 
-    def launch_epie_group(self, g, *, update_object, update_probe, update_position, calc_error):
-        """epie_run (solvers.py:415-504) for group slot g."""
+    def launch_epie_group(self, g, *, update_object, update_probe, update_position, calc_error, constraints=()):
+        """epie_run (solvers.py:415-504) for group slot g, then its group constraints
+        (solvers.py:367-371)."""
         d, st = self.desc_refs[g], stream_handle()
         c = self.call
         self.launch_forward(g, calc_error, illum=False)
@@ -406,6 +428,7 @@ class GroupWorkspace:
                 c("phx_epie_object_apply", d, i, st)
         if update_probe:
             c("phx_epie_probe_apply", d, st)
+        self._launch_constraints(constraints, g, st, st)
 
     def scale_probe(self, factor):
         _lib.call("phx_scale_c64", _ptr(self.probe), self.probe.numel(), float(factor), stream_handle())

@@ -17,10 +17,11 @@ import logging
 
 import numpy
 
+from ..constraints import make_constraints
 from ..device import GroupWorkspace, host_patterns
 from ..groups import GroupManager, mask_fraction_of_groups
 from ..optics import make_propagators
-from ..plan import process_flag, process_schedule
+from ..plan import process_flag, process_schedule, shuffle_groups_flag
 from .position import make_position_solver
 
 logger = logging.getLogger(__name__)
@@ -34,6 +35,15 @@ class Observer:
     def update_group(self, state, force=False): pass
     def update_iteration(self, state, i, n, error=None): pass
     def finish_engine(self, state): pass
+
+
+def wants_group_updates(observer):
+    """Whether `observer` needs `update_group` after every group with that group's state
+    (its own `wants_groups`, else whether it overrides this module's no-op)."""
+    if hasattr(observer, 'wants_groups'):
+        return bool(observer.wants_groups)
+    fn = getattr(type(observer), 'update_group', None)
+    return fn is not None and fn is not Observer.update_group
 
 
 class SimulationState:
@@ -73,6 +83,8 @@ class _SolverBase:
         self.engine_plan = engine_plan
         self.plan = props
         self.graphs = _GraphCache(use_graphs)
+        # device group constraints (phaser_amd.constraints), applied after every group
+        self.group_constraints = ()
 
     def init(self, sim):
         return sim
@@ -107,19 +119,33 @@ class _SolverBase:
         G = len(groups)
         err_groups = [g for g in range(G) if calc_error and calc_error_mask[g]]
 
-        def launch():
+        def launch_group(g):
+            self._launch_group(ws, g, update_object=update_object, update_probe=update_probe,
+                               update_position=update_positions, calc_error=(g in err_groups))
+
+        if wants_group_updates(observer):
+            # the observer sees the state after every group (solvers.py:109-119): groups run
+            # one by one, each followed by the finite check and the callback
+            ws.check_state_finite(0)
             for g in range(G):
-                self._launch_group(ws, g, update_object=update_object, update_probe=update_probe,
-                                   update_position=update_positions, calc_error=(g in err_groups))
+                launch_group(g)
+                bad = ws.finite_group()
+                if bad is not None:
+                    raise ValueError(f"NaN or inf encountered in object or probe, group {bad}")
+                observer.update_group(sim.state, self.engine_plan.send_every_group)
+        else:
+            def launch():
+                ws.check_state_finite(0)
+                for g in range(G):
+                    launch_group(g)
 
-        key = (bool(update_object), bool(update_probe), bool(update_positions), bool(calc_error))
-        self.graphs.run(key, launch)
-
-        bad = ws.finite_group()
-        if bad is not None:
-            raise ValueError(f"NaN or inf encountered in object or probe, group {bad}")
-        for _g in range(G):
-            observer.update_group(sim.state, self.engine_plan.send_every_group)
+            key = (bool(update_object), bool(update_probe), bool(update_positions), bool(calc_error))
+            self.graphs.run(key, launch)
+            # the update kernels record the first group whose object or probe turned
+            # non-finite; the check is made once the iteration's launches have run
+            bad = ws.finite_group()
+            if bad is not None:
+                raise ValueError(f"NaN or inf encountered in object or probe, group {bad}")
 
         if update_positions:
             pos_update = ws.pos_update.cpu().numpy().reshape(*ws.scan_shape, 2)
@@ -146,7 +172,7 @@ class LSQMLSolver(_SolverBase):
             gamma=self._schedule('gamma', sim))
 
     def _launch_group(self, ws, g, **flags):
-        ws.launch_lsqml_group(g, **flags)
+        ws.launch_lsqml_group(g, constraints=self.group_constraints, **flags)
 
 
 class EPIESolver(_SolverBase):
@@ -161,7 +187,7 @@ class EPIESolver(_SolverBase):
                                  beta_probe=self._schedule('beta_probe', sim))
 
     def _launch_group(self, ws, g, **flags):
-        ws.launch_epie_group(g, **flags)
+        ws.launch_epie_group(g, constraints=self.group_constraints, **flags)
 
 
 SOLVER_CLASSES = {'lsqml': LSQMLSolver, 'epie': EPIESolver}
@@ -187,8 +213,6 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
 
-    if props.group_constraints or props.iter_constraints:
-        raise NotImplementedError("group/iter constraints are not implemented by the MI355X backend")
     noise = props.noise_model
     if noise.name == 'poisson':
         raise NotImplementedError()  # PoissonNoiseModel.calc_wave_update (noise_models.py:104-111)
@@ -197,7 +221,7 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     update_object = process_flag(props.update_object)
     update_positions = process_flag(props.update_positions)
     calc_error = process_flag(props.calc_error)
-    shuffle_groups = process_flag(props.shuffle_groups if props.shuffle_groups is not None else (not props.compact))
+    shuffle_groups = shuffle_groups_flag(props)
 
     patterns = host_patterns(args['data'].patterns)   # numpy, or float32 already on the GPU
     pattern_mask = numpy.asarray(args['data'].pattern_mask)
@@ -207,7 +231,7 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     groups = GroupManager(state.scan, props.grouping, props.compact, seed=seed)
     calc_error_mask = mask_fraction_of_groups(len(groups), props.calc_error_fraction)
     position_solver = None if props.position_solver is None else make_position_solver(props.position_solver)
-    position_solver_state = None if position_solver is None else position_solver.init_state(state.scan)
+    position_solver_state = None if position_solver is None else position_solver.init_state(state)
 
     observer.init_engine(state, recons_name=args.get('recons_name', ''), plan=props, noise_model=noise.name)
     start_i = state.iter.total_iter
@@ -220,6 +244,9 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
                         keep_chi=(props.solver.name == 'lsqml') if keep_chi is None else bool(keep_chi),
                         fused=fused)
     ws.set_positions(state.object.sampling, state.scan)
+    # constraint tables are built once here (simulation.py:127-132), outside any graph
+    solver.group_constraints = tuple(make_constraints(props.group_constraints, ws, state, 'group constraint'))
+    iter_constraints = make_constraints(props.iter_constraints, ws, state, 'iter constraint')
     sim = SimulationState(state=state, noise_model=noise, workspace=ws)
     sim = solver.init(sim)
 
@@ -240,6 +267,11 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
             update_object=update_object(fargs), update_probe=update_probe(fargs),
             update_positions=iter_update_positions, calc_error=calc_error(fargs),
             calc_error_mask=calc_error_mask, observer=observer)
+
+        # iter constraints (run.py:93); a NaN they make is found by the next iteration's
+        # state check, where the reference's next finite check finds it
+        for con in iter_constraints:
+            con.apply(ws)
 
         if iter_update_positions:
             if not position_solver:

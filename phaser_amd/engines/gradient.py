@@ -20,10 +20,11 @@ import logging
 import numpy
 
 from .. import _lib
+from ..constraints import CostRegularizers, make_constraints
 from ..device import GroupWorkspace, host_patterns, stream_handle, INT32_MAX
 from ..groups import GroupManager
 from ..optics import make_propagators
-from ..plan import process_flag, process_schedule
+from ..plan import process_flag, process_schedule, shuffle_groups_flag
 from .conventional import Observer
 
 logger = logging.getLogger(__name__)
@@ -340,8 +341,6 @@ def run_engine(args, props, *, recompute=False):
     seed = args.get('seed')
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
-    if props.regularizers or props.group_constraints or props.iter_constraints:
-        raise NotImplementedError("regularizers/constraints are not implemented by the MI355X backend")
     noise = props.noise_model
     all_vars, group_solvers, iter_solvers = process_solvers(props)
     if 'tilt' in all_vars:
@@ -349,7 +348,7 @@ def run_engine(args, props, *, recompute=False):
 
     flags = {'probe': process_flag(props.update_probe), 'object': process_flag(props.update_object),
              'positions': process_flag(props.update_positions), 'tilt': process_flag(props.update_tilt)}
-    shuffle_groups = process_flag(props.shuffle_groups if props.shuffle_groups is not None else (not props.compact))
+    shuffle_groups = shuffle_groups_flag(props)
     groups = GroupManager(state.scan, props.grouping, props.compact, seed)
     patterns = host_patterns(args['data'].patterns)   # numpy, or float32 already on the GPU
     mask = numpy.asarray(args['data'].pattern_mask)
@@ -365,6 +364,9 @@ def run_engine(args, props, *, recompute=False):
                         psi_slots=3 if recompute else 0, gradients=True, keep_chi=not recompute)
     ws.set_positions(state.object.sampling, state.scan)
     model = MultisliceModel(ws, poisson=(noise.name == 'poisson'))
+    regularizers = CostRegularizers(props.regularizers, ws.S)
+    group_constraints = make_constraints(props.group_constraints, ws, state, 'group constraint')
+    iter_constraints = make_constraints(props.iter_constraints, ws, state, 'iter constraint')
 
     # dry run: rescale the probe (gradient/run.py:196-209)
     glist = list(groups.iter(state.scan))
@@ -393,6 +395,7 @@ def run_engine(args, props, *, recompute=False):
         glist = list(groups.iter(state.scan, i, shuffle_groups(fargs)))
         ws.set_groups(glist)
         ws.reset_finite_flag()
+        ws.check_state_finite(0)
         want_obj, want_probe, want_pos = ('object' in iter_vars), ('probe' in iter_vars), ('positions' in iter_vars)
         for g in range(G):
             # the probe spectra change only with the probe
@@ -401,6 +404,9 @@ def run_engine(args, props, *, recompute=False):
                            next_group=(g + 1 if FOLD_GROUP_BEGIN and g + 1 < G else None))
             bsz = float(ws.sizes_host[g])
             loss_ptr = ctypes.c_void_p(ws.red[g].data_ptr() + 8 * model.L['loss'])
+            # cost regularizers on the pre-update object (run.py:399-403): their loss joins
+            # the group loss, their gradient the object's
+            regularizers.apply(ws, int(ws.sizes_host[g]), loss_ptr, want_obj)
             for s in group_solvers:
                 items = []
                 if 'object' in s.params and want_obj:
@@ -409,6 +415,10 @@ def run_engine(args, props, *, recompute=False):
                     items.append(('probe', ws.probe, ws.grad_probe.view(ws.probe.shape), bsz))
                 if items:
                     s.step_device(items, loss_ptr, sqnorm, ws.finite_flag, g)
+            # group constraints after the solver steps; the reference's finite check
+            # follows them (run.py:344-347, :261), so a NaN they make is group g's
+            for con in group_constraints:
+                con.apply(ws, g, g)
             observer.update_group(state, props.send_every_group)
 
         bad = ws.finite_group()
@@ -434,6 +444,10 @@ def run_engine(args, props, *, recompute=False):
                 state.scan += upd.reshape(state.scan.shape)
             state.object.sampling.check_scan(state.scan, state.probe.sampling.extent / 2.)
             ws.set_positions(state.object.sampling, state.scan)
+
+        # iter constraints (run.py:276-279)
+        for con in iter_constraints:
+            con.apply(ws)
 
         state.progress.iters = numpy.concatenate([state.progress.iters, [i + start_i]])
         state.progress.detector_errors = numpy.concatenate([state.progress.detector_errors, [loss]])

@@ -63,9 +63,68 @@ def plan_from_phaser(plan_data, engine_type):
     return from_dict(d)
 
 
-def run_phaser_engine(args, plan_data, engine_type):
+def _overrides(obs, name):
+    """Whether phaser observer `obs` does anything in callback `name`: a method of its
+    own, not the no-op of phaser's base `Observer` (observer.py:18-69).  phaser's
+    ObserverSet (observer.py:271-332) forwards to its children, so they decide."""
+    children = getattr(obs, 'inner', None)
+    if isinstance(children, (tuple, list)):
+        return any(_overrides(c, name) for c in children)
+    fn = getattr(type(obs), name, None)
+    return fn is not None and getattr(fn, '__qualname__', '') != f'Observer.{name}'
+
+
+class PhaserObserver:
+    """The engines' observer callbacks forwarded to one of phaser's observers
+    (LoggingObserver, PatienceObserver, SaveObserver, an ObserverSet of them; observer.py)
+    with phaser's own state object: before each callback the engine's state is copied
+    into `target` (state_into_phaser), so the observer sees the arrays, iteration counters
+    and progress it would see on the numpy backend.  `plan` is phaser's engine plan object,
+    handed to `init_engine` (SaveObserver reads its save flags and options there).
+
+    An EarlyTermination raised by the observer (PatienceObserver, observer.py:142-181)
+    propagates unchanged, carrying `target`, which is already up to date.  Per-group
+    copies are made only when the observer implements `update_group`
+    (`wants_groups`); the engines then call it after every group."""
+
+    def __init__(self, inner, target, plan=None):
+        self.inner = inner
+        self.target = target
+        self.plan = plan
+        self.wants_groups = _overrides(inner, 'update_group')
+        self._wants_iter = _overrides(inner, 'update_iteration')
+
+    def _sync(self, state):
+        return state_into_phaser(state.to_numpy() if hasattr(state, 'to_numpy') else state, self.target)
+
+    def init_engine(self, state, *, recons_name, plan, **kwargs):
+        self.inner.init_engine(self._sync(state), recons_name=recons_name,
+                               plan=self.plan if self.plan is not None else plan, **kwargs)
+
+    def start_engine(self, state):
+        self.inner.start_engine(self._sync(state))
+
+    def heartbeat(self):
+        if hasattr(self.inner, 'heartbeat'):
+            self.inner.heartbeat()
+
+    def update_group(self, state, force=False):
+        if self.wants_groups:
+            self.inner.update_group(self._sync(state), force)
+
+    def update_iteration(self, state, i, n, error=None):
+        self.inner.update_iteration(self._sync(state) if self._wants_iter else self.target, i, n, error)
+
+    def finish_engine(self, state):
+        self.inner.finish_engine(self._sync(state))
+
+
+def run_phaser_engine(args, plan_data, engine_type, props=None):
     """Run one engine on phaser's EngineArgs and plan mapping; returns phaser's state
-    object, updated in place (the contract of phaser/hooks/__init__.py:174-185)."""
+    object, updated in place (the contract of phaser/hooks/__init__.py:174-185).
+
+    phaser's observer (`args['observer']`) receives the engine's callbacks through
+    PhaserObserver; `props` is phaser's plan object, given to its `init_engine`."""
     if engine_type == 'conventional':
         from .engines.conventional import run_engine
     elif engine_type == 'gradient':
@@ -75,6 +134,7 @@ def run_phaser_engine(args, plan_data, engine_type):
     ours = dict(args)
     ours['state'] = state_from_phaser(args['state'])
     ours['data'] = patterns_from_phaser(args['data'])
-    ours['observer'] = None   # phaser's observer expects its own state class
+    obs = args.get('observer')
+    ours['observer'] = None if obs is None else PhaserObserver(obs, args['state'], props)
     out = run_engine(ours, plan_from_phaser(plan_data, engine_type))
     return state_into_phaser(out.to_numpy(), args['state'])

@@ -32,6 +32,13 @@ def process_flag(flag):
     return flag
 
 
+def shuffle_groups_flag(props):
+    """Whether groups are reshuffled each iteration: `shuffle_groups or not compact`
+    (conventional/run.py:32, gradient/run.py:185), so an explicit `False` still shuffles
+    sparse groups, and compact groups keep their order unless a flag says otherwise."""
+    return process_flag(props.shuffle_groups or not props.compact)
+
+
 def process_schedule(schedule):
     """float or callable(args) -> float (types.py:163-167); dict forms of the known
     schedules (constant / piecewise / expr, hooks/schedule.py:33-75) are accepted."""
@@ -124,6 +131,99 @@ class MomentumPositionSolverPlan:
 POSITION_SOLVERS = {'steepest_descent': SteepestDescentPositionSolverPlan, 'momentum': MomentumPositionSolverPlan}
 
 
+# -------------------------------------------- constraints and cost regularizers
+# (phaser/hooks/regularization.py:36-118: names, fields and defaults)
+@dataclass
+class ClampObjectAmplitudePlan:
+    amplitude: float = 1.1
+    name: str = 'clamp_object_amplitude'
+
+
+@dataclass
+class LimitProbeSupportPlan:
+    max_angle: float = None
+    name: str = 'limit_probe_support'
+
+    def __post_init__(self):
+        if self.max_angle is None:
+            raise ValueError("limit_probe_support needs 'max_angle' (mrad)")
+
+
+@dataclass
+class RegularizeLayersPlan:
+    sigma: float = 50.0
+    weight: float = 0.9
+    name: str = 'layers'
+
+
+@dataclass
+class ObjLowPassPlan:
+    max_freq: float = 0.4
+    name: str = 'obj_low_pass'
+
+
+@dataclass
+class ObjGaussianPlan:
+    sigma: float = None
+    weight: float = 0.9
+    name: str = 'obj_gaussian'
+
+    def __post_init__(self):
+        if self.sigma is None:
+            raise ValueError("obj_gaussian needs 'sigma' (angstrom)")
+
+
+@dataclass
+class CostRegularizerPlan:
+    cost: float = None
+    eps: float = 1.0e-8
+    name: str = ''
+
+    def __post_init__(self):
+        if self.cost is None:
+            raise ValueError(f"cost regularizer {self.name!r} needs 'cost'")
+
+
+@dataclass
+class UnsupportedPlan:
+    """A constraint or regularizer phaser knows but this backend does not run; engines
+    raise NotImplementedError naming it."""
+    name: str = ''
+    props: t.Any = None
+
+
+GROUP_CONSTRAINTS = {'clamp_object_amplitude': ClampObjectAmplitudePlan, 'limit_probe_support': LimitProbeSupportPlan,
+                     'obj_low_pass': ObjLowPassPlan, 'obj_gaussian': ObjGaussianPlan}
+ITER_CONSTRAINTS = dict(GROUP_CONSTRAINTS, layers=RegularizeLayersPlan)
+REGULARIZER_NAMES = {'obj_l1': 'obj_l1', 'obj_l2': 'obj_l2', 'obj_tikh': 'obj_tikh', 'obj_tikhonov': 'obj_tikh',
+                     'layers_tv': 'layers_tv', 'layers_tikh': 'layers_tikh', 'layers_tikhonov': 'layers_tikh'}
+# known to phaser (hooks/regularization.py:76-118), not run by this backend
+UNSUPPORTED = {'opr_gaussian', 'tilt_gaussian', 'remove_phase_ramp', 'obj_phase_l1', 'obj_recip_l1', 'obj_tv',
+               'probe_phase_tikh', 'probe_phase_tikhonov', 'probe_recip_tv', 'probe_recip_tikh',
+               'probe_recip_tikhonov'}
+
+
+def _resolve_item(val, known, what):
+    if not isinstance(val, (str, dict)):
+        return val
+    d = {'type': val} if isinstance(val, str) else dict(val)
+    kind = d.get('type')
+    if what == 'regularizer' and kind in REGULARIZER_NAMES:
+        d.pop('type')
+        bad = set(d) - {'cost', 'eps'}
+        if bad:
+            raise ValueError(f"Unknown {what} options {sorted(bad)}")
+        return CostRegularizerPlan(name=REGULARIZER_NAMES[kind], **d)
+    if kind in UNSUPPORTED:
+        d.pop('type')
+        return UnsupportedPlan(kind, d)
+    return _resolve(d, known, what)
+
+
+def _resolve_list(vals, known, what):
+    return [_resolve_item(v, known, what) for v in (vals or [])]
+
+
 @dataclass
 class EnginePlan:
     """phaser/plan.py:43-83"""
@@ -154,8 +254,7 @@ class EnginePlan:
 
 @dataclass
 class ConventionalEnginePlan(EnginePlan):
-    """phaser/plan.py:126-132.  `group_constraints` / `iter_constraints` must be empty
-    lists here (constraint kernels are not part of this backend)."""
+    """phaser/plan.py:126-132."""
     noise_model: t.Any = None
     solver: t.Any = None
     position_solver: t.Any = None
@@ -167,6 +266,8 @@ class ConventionalEnginePlan(EnginePlan):
         self.solver = _resolve(self.solver, SOLVERS, 'solver')
         if self.position_solver is not None:
             self.position_solver = _resolve(self.position_solver, POSITION_SOLVERS, 'position_solver')
+        self.group_constraints = _resolve_list(self.group_constraints, GROUP_CONSTRAINTS, 'group constraint')
+        self.iter_constraints = _resolve_list(self.iter_constraints, ITER_CONSTRAINTS, 'iter constraint')
 
 
 @dataclass
@@ -217,6 +318,9 @@ class GradientEnginePlan(EnginePlan):
             key = frozenset([k]) if isinstance(k, str) else frozenset(k)
             solvers[key] = _resolve(v, GRADIENT_SOLVERS, 'solver')
         self.solvers = solvers
+        self.regularizers = _resolve_list(self.regularizers, {}, 'regularizer')
+        self.group_constraints = _resolve_list(self.group_constraints, GROUP_CONSTRAINTS, 'group constraint')
+        self.iter_constraints = _resolve_list(self.iter_constraints, ITER_CONSTRAINTS, 'iter constraint')
 
 
 def _resolve(val, known, what):
@@ -229,6 +333,8 @@ def _resolve(val, known, what):
     if isinstance(val, dict):
         d = dict(val)
         kind = d.pop('type')
+        if kind not in known:
+            raise ValueError(f"Unknown {what} '{kind}'")
         cls = known[kind]
         names = {f.name for f in fields(cls)}
         bad = set(d) - names

@@ -351,3 +351,37 @@ def test_update_launch_clearing_next_group_matches_separate_clear(monkeypatch):
         assert noise <= 1e-7
         assert e <= 1e-7
         numpy.testing.assert_allclose(a.progress.detector_errors, b.progress.detector_errors, rtol=1e-6)
+
+
+@pytest.mark.parametrize('solver,per_group', [('lsqml', False), ('lsqml', True), ('epie', False)])
+def test_nan_in_state_raises_reference_error(solver, per_group):
+    """A NaN already in the object (in the last slice, which the LSQ-ML/ePIE dry runs skip,
+    so the presolve stays finite): the reference's check after group 0 sees the whole
+    object (solvers.py:109) and raises for group 0.  The GPU engine raises the same
+    ValueError, with graphs and with an observer that takes every group's state."""
+    from phaser_amd.engines.conventional import run_engine, Observer
+    from phaser_amd.plan import ConventionalEnginePlan
+    _, state, pats = _problem(n=32, n_modes=1, n_slices=2, scan_shape=(6, 6), scan_step_px=3.0, jitter_px=0.5,
+                              seed=11)
+    state.object.data = numpy.array(state.object.data)
+    state.object.data[1, 2, 2] = numpy.nan
+    o = to_oracle_state(state)
+    with pytest.raises(ValueError) as oe:
+        oeng.run_engine(o, pats.patterns, pats.pattern_mask, solver=solver, noise_model=onoise.AmplitudeNoiseModel(),
+                        niter=2, grouping=12, seed=7)
+
+    class EveryGroup(Observer):
+        def __init__(self):
+            self.groups = 0
+
+        def update_group(self, state, force=False):
+            self.groups += 1
+    obs = EveryGroup() if per_group else None
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver=solver, niter=2, grouping=12)
+    with pytest.raises(ValueError) as ge:
+        run_engine({'data': pats, 'state': state.copy(), 'dtype': numpy.float32, 'seed': 7, 'observer': obs,
+                    'recons_name': 't'}, plan)
+    print(f"oracle: {oe.value}; gpu: {ge.value}", flush=True)
+    assert str(ge.value) == str(oe.value) == "NaN or inf encountered in object or probe, group 0"
+    if per_group:
+        assert obs.groups == 0     # raised before the observer saw group 0's state

@@ -136,3 +136,65 @@ def test_optimizer_restatements_closed_forms():
     p = og.Solver('polyak_sgd', ['object'], scaling=1.0, max_learning_rate=10.0, f_min=0.0)
     up = p.update({'object': numpy.array([3.0, 4.0])}, 5.0)['object']
     numpy.testing.assert_allclose(up, numpy.array([3.0, 4.0]) * (5.0 / 25.0))
+
+
+@pytest.mark.parametrize('nesterov', [False, True])
+def test_adam_three_steps_against_explicit_sums(nesterov):
+    """optax scale_by_adam (gradient/solvers.py:117-127) over three steps with b1 != b2,
+    eps_root != 0 and complex directions, against the moments written as explicit sums
+    m_t = (1 - b1) sum_k b1^(t-k) g_k, v_t = (1 - b2) sum_k b2^(t-k) |g_k|^2, the bias
+    corrections 1 - b^t and (nesterov) m_hat = b1 m_t/(1 - b1^(t+1)) + (1 - b1) g_t/(1 - b1^t)."""
+    b1, b2, eps, eps_root, lr = 0.8, 0.95, 1e-3, 0.25, 0.3
+    rng = numpy.random.default_rng(2)
+    gs = [rng.normal(size=4) + 1j * rng.normal(size=4) for _ in range(3)]
+    a = og.Solver('adam', ['object'], learning_rate=lr, b1=b1, b2=b2, eps=eps, eps_root=eps_root, nesterov=nesterov)
+    for t in range(1, 4):
+        got = a.update({'object': gs[t - 1]}, 0.0)['object']
+        m = (1 - b1) * sum(b1**(t - k) * gs[k - 1] for k in range(1, t + 1))
+        v = (1 - b2) * sum(b2**(t - k) * numpy.abs(gs[k - 1])**2 for k in range(1, t + 1))
+        if nesterov:
+            mh = b1 * m / (1 - b1**(t + 1)) + (1 - b1) * gs[t - 1] / (1 - b1**t)
+        else:
+            mh = m / (1 - b1**t)
+        want = lr * mh / (numpy.sqrt(v / (1 - b2**t) + eps_root) + eps)
+        numpy.testing.assert_allclose(got, want, rtol=1e-12)
+    # a constant direction: m_hat = g, v_hat = |g|^2 at every step (no nesterov)
+    c = og.Solver('adam', ['object'], learning_rate=1.0, b1=b1, b2=b2, eps=0.0, eps_root=0.0)
+    for _ in range(3):
+        u = c.update({'object': numpy.array([3 - 4j])}, 0.0)['object']
+        numpy.testing.assert_allclose(u, [(3 - 4j) / 5], rtol=1e-12)
+
+
+def test_gradient_engine_step_direction_and_scale_by_finite_differences():
+    """One iteration, one group of all B positions, plain SGD with lr = 1: the object moves
+    by -g/(probe_int B) and (object frozen) the probe by -g/B, g = dL/dRe + i dL/dIm from
+    central finite differences of the FP64 loss at the rescaled state (gradient/run.py:
+    324-330 with jax's -conj(g_jax) = -g).  Independent of model_grad."""
+    import copy
+    state, pats, mask, nz, ky, kx, props, group = _small()
+    B = group.shape[-1]
+    rng = numpy.random.default_rng(4)
+    S, H, W = state.object.data.shape
+    for var in ('object', 'probe'):
+        st = copy.deepcopy(state)
+        before = copy.deepcopy(state)
+        og.run_engine(st, pats, mask, noise=nz, solvers=[og.Solver('sgd', [var], learning_rate=1.0)], niter=1,
+                      grouping=B, seed=1, dtype=numpy.float64, update_object=(var == 'object'),
+                      update_probe=(var == 'probe'))
+        # the engine's dry-run rescale (gradient/run.py:196-209)
+        factors = og.dry_run(before, group, pats, ky, kx, props)
+        before.probe.data = before.probe.data * numpy.sqrt(numpy.mean(factors))
+        probe_int = numpy.sum(numpy.abs(before.probe.data)**2)
+        arr_b = before.object.data if var == 'object' else before.probe.data
+        arr_a = st.object.data if var == 'object' else st.probe.data
+        scale = probe_int * B if var == 'object' else B
+        f = lambda: og.model_loss(before, group, pats, mask, nz, ky, kx, props)  # noqa: E731
+        for _ in range(6):
+            if var == 'object':
+                idx = (int(rng.integers(0, S)), int(rng.integers(H // 2 - 4, H // 2 + 4)),
+                       int(rng.integers(W // 2 - 4, W // 2 + 4)))
+            else:
+                idx = tuple(int(rng.integers(0, d)) for d in arr_b.shape)
+            want = -_fd_complex(f, arr_b, idx, h=1e-6) / scale
+            got = arr_a[idx] - arr_b[idx]
+            assert abs(got - want) <= 1e-5 * abs(want) + 1e-14, (var, idx, got, want)

@@ -7,8 +7,16 @@ from numpy.testing import assert_array_almost_equal, assert_array_equal
 from oracle import misc, num, object as oobj, optics, physics_scan
 
 
+# The groupings KATs run on the oracle's restatement and on the product's own copy
+# (phaser_amd/groups.py), which the engines use for their group order.
+def _groups_modules():
+    import phaser_amd.groups as pgroups
+    return [misc, pgroups]
+
+
 # --- tests/test_misc.py:7-31 (sparse groupings, seed 'seed1') -----------------------
-def test_sparse_groupings_known_answer():
+@pytest.mark.parametrize('misc', _groups_modules(), ids=['oracle', 'product'])
+def test_sparse_groupings_known_answer(misc):
     assert [a.tolist() for a in misc.create_sparse_groupings(32, 8, 'seed1')] == [
         [[23, 31, 0, 30, 29, 3, 4, 15]],
         [[14, 6, 9, 28, 13, 18, 17, 26]],
@@ -29,7 +37,8 @@ def test_sparse_groupings_known_answer():
 
 
 # --- tests/test_misc.py:33-49 (compact groupings) -----------------------------------
-def test_compact_groupings_known_answer():
+@pytest.mark.parametrize('misc', _groups_modules(), ids=['oracle', 'product'])
+def test_compact_groupings_known_answer(misc):
     scan = numpy.stack(numpy.meshgrid(numpy.linspace(0., 20., 8), numpy.linspace(0., 20., 8),
                                       indexing='ij'), axis=-1).reshape(-1, 2)
     assert [a.tolist() for a in misc.create_compact_groupings(scan, 16, 'seed1')] == [
@@ -40,7 +49,8 @@ def test_compact_groupings_known_answer():
     ]
 
 
-def test_mask_fraction_of_groups():
+@pytest.mark.parametrize('misc', _groups_modules(), ids=['oracle', 'product'])
+def test_mask_fraction_of_groups(misc):
     # every floor(n/k)-th group starting at 0, k = ceil(n * fraction)
     assert misc.mask_fraction_of_groups(64, 0.1).nonzero()[0].tolist() == [0, 9, 18, 27, 36, 45, 54, 63]
     assert misc.mask_fraction_of_groups(4, 0.1).nonzero()[0].tolist() == [0]
