@@ -102,3 +102,83 @@ def scramble(a: numpy.ndarray) -> numpy.ndarray:
 def twiddles(n: int) -> numpy.ndarray:
     """exp(-2 pi i j / n), j < n, rounded from FP64 to complex64."""
     return numpy.exp(-2j * numpy.pi * numpy.arange(n) / n).astype(numpy.complex64)
+
+
+def hermetian_modes(base_probe, y_orders, x_orders):
+    """Hermite-Gauss-like modes y^a x^b P about the probe's intensity centroid, each
+    orthogonalised against the earlier ones (with the reference's projection
+    sum(prev conj(mode))) and normalised (optics.py:97-137)."""
+    real = numpy.float32 if base_probe.dtype == numpy.complex64 else numpy.float64
+    y_orders, x_orders = numpy.broadcast_arrays(y_orders, x_orders)
+    yy, xx = numpy.indices(base_probe.shape, dtype=real)
+    mag = base_probe.real**2 + base_probe.imag**2
+    com_y, com_x = (numpy.sum(a * mag) / numpy.sum(mag) for a in (yy, xx))
+    yy -= com_y
+    xx -= com_x
+    modes = numpy.empty((y_orders.size, *base_probe.shape), dtype=base_probe.dtype)
+    for i, (yp, xp) in enumerate(zip(y_orders.flat, x_orders.flat)):
+        mode = yy**yp * xx**xp * base_probe
+        for j in range(i):
+            mode -= modes[j] * numpy.sum(modes[j] * numpy.conj(mode))
+        mode /= numpy.sqrt(numpy.sum(mode.real**2 + mode.imag**2))
+        modes[i] = mode
+    return modes.reshape((*y_orders.shape, *base_probe.shape))
+
+
+def make_hermetian_modes(base_probe, n_modes, base_mode_power=0.7, rel_powers=1.0):
+    """`n_modes` probe modes from `base_probe` (optics.py:54-94): the base mode carries
+    `base_mode_power`, the others split the rest by `rel_powers`; mode i sits on
+    anti-diagonal d = ceil((sqrt(8i + 9) - 3)/2) of the (y order, x order) grid."""
+    if not (0.0 <= base_mode_power < 1.0):
+        raise ValueError(f"Invalid base_mode_power '{base_mode_power}'. Expected a value between 0 and 1")
+    real = numpy.float32 if base_probe.dtype == numpy.complex64 else numpy.float64
+    rel = numpy.array(rel_powers, dtype=real).ravel()
+    rel = numpy.pad(rel, (0, n_modes - rel.size - 1), mode='edge')[:n_modes - 1]
+    powers = numpy.concatenate([[base_mode_power], (1.0 - base_mode_power) * rel])
+    i = numpy.arange(n_modes)
+    diag = numpy.ceil((numpy.sqrt(8 * i + 9) - 3) / 2).astype(numpy.int_)
+    xx = (i - diag * (diag + 1) / 2).astype(numpy.int_)
+    yy = diag - xx
+    modes = hermetian_modes(base_probe, yy, xx)
+    modes *= numpy.array(numpy.sqrt(powers)[:, None, None], dtype=modes.dtype)
+    return modes
+
+
+def resample_slices(obj, old_thicknesses, new_thicknesses):
+    """Re-slice an object along z keeping the mean potential (object.py:56-158 with its
+    defaults: centres aligned, vacuum padding): log(O) per unit thickness interpolated
+    linearly between slice centres, times the new thickness, exponentiated.  Multi to
+    single slice is the product; single to multi splits log(O) by thickness fraction."""
+    obj = numpy.asarray(obj)
+    while obj.ndim < 3:
+        obj = obj[None]
+    old_t = numpy.asarray(old_thicknesses, dtype=numpy.float64)
+    new_t = numpy.asarray(new_thicknesses, dtype=numpy.float64)
+    if len(new_t) < 2:
+        return obj if obj.shape[0] == 1 else numpy.prod(obj, axis=0, keepdims=True)
+    if obj.shape[0] == 1:
+        frac = (new_t / numpy.sum(new_t))[:, None, None]
+        return numpy.exp((numpy.log(obj) * frac).astype(obj.dtype))
+    if obj.shape[0] != len(old_t):
+        raise ValueError(f"Expected an object with {len(old_t)} slices, instead got object shape {obj.shape}")
+    old_zs = numpy.cumsum(old_t) - old_t / 2.
+    new_zs = numpy.cumsum(new_t) - new_t / 2. + (numpy.sum(old_t) - numpy.sum(new_t)) / 2.
+    pot = (numpy.log(obj) / old_t[:, None, None]).astype(obj.dtype)
+    zero = numpy.zeros_like(pot[0])
+    old_zs = numpy.concatenate(([old_zs[0] - old_t[0]], old_zs, [old_zs[-1] + old_t[-1]]))
+    pot = numpy.concatenate((zero[None], pot, zero[None]), axis=0)
+    real = numpy.float32 if obj.dtype == numpy.complex64 else numpy.float64
+    idx = numpy.searchsorted(old_zs, new_zs) - 1
+    dz = numpy.diff(old_zs)
+    new = numpy.empty((len(new_zs), *pot.shape[1:]), dtype=pot.dtype)
+    for i, z in enumerate(new_zs):
+        if z <= old_zs[0]:
+            new[i] = pot[0]
+        elif old_zs[-1] <= z:
+            new[i] = pot[-1]
+        else:
+            k = idx[i]
+            t = numpy.array(float((z - old_zs[k]) / dz[k]), dtype=real)
+            new[i] = ((1 - t) * pot[k] + t * pot[k + 1]).astype(pot.dtype)
+    new *= new_t[:, None, None]
+    return numpy.exp(new.astype(obj.dtype))

@@ -250,6 +250,7 @@ class EnginePlan:
     early_termination: t.Optional[int] = None
     early_termination_smoothing: float = 0.9
     send_every_group: bool = False
+    save_options: t.Any = None
 
 
 @dataclass
@@ -352,3 +353,94 @@ def from_dict(d: t.Mapping[str, t.Any]):
         d['calc_error'] = Flag(**d['calc_error'])
     cls = {'conventional': ConventionalEnginePlan, 'gradient': GradientEnginePlan}[kind]
     return cls(**d)
+
+
+# ------------------------------------------------------------------ reconstruction plans
+@dataclass
+class SaveOptions:
+    """phaser/plan.py:29-40."""
+    images: t.Tuple[str, ...] = ('probe', 'object_phase_stack')
+    crop_roi: bool = True
+    unwrap_phase: bool = False
+    img_dtype: str = '16bit'
+    plot_ext: str = 'svg'
+    plot_dpi: int = 300
+    out_dir: str = '{name}'
+    img_fmt: str = '{type}_iter{iter.total_iter:03}.{ext}'
+    hdf5_fmt: str = 'iter{iter.total_iter:03}.h5'
+
+
+def slice_thicknesses(slices):
+    """The thickness list of a plan's `slices` entry (types.py:95-121): {thicknesses},
+    {n, slice_thickness} or {n, total_thickness}."""
+    if slices is None:
+        return None
+    if isinstance(slices, (list, tuple)):
+        return [float(v) for v in slices]
+    s = dict(slices)
+    if 'thicknesses' in s:
+        return [float(v) for v in s['thicknesses']]
+    n = int(s['n'])
+    if 'slice_thickness' in s:
+        return [float(s['slice_thickness'])] * n
+    if 'total_thickness' in s:
+        return [float(s['total_thickness']) / n] * n
+    raise ValueError(f"Invalid slices {slices!r}")
+
+
+@dataclass
+class ReconsPlan:
+    """phaser/plan.py:176-198: a whole reconstruction (data, initial state, engines)."""
+    name: str = None
+    raw_data: t.Any = None
+    engines: t.List[t.Any] = field(default_factory=list)
+    file_type: str = 'phaser_plan'
+    version: str = '1.0'
+    backend: t.Optional[str] = None
+    dtype: str = 'float32'
+    wavelength: t.Optional[float] = None
+    post_load: t.List[t.Any] = field(default_factory=list)
+    init: t.Dict[str, t.Any] = field(default_factory=dict)
+    post_init: t.List[t.Any] = field(default_factory=list)
+    slices: t.Any = None
+
+    def __post_init__(self):
+        if self.file_type != 'phaser_plan':
+            raise ValueError(f"Expected file_type 'phaser_plan', got {self.file_type!r}")
+        if str(self.version) != '1.0':
+            raise ValueError(f"Unsupported plan version {self.version!r} (expected '1.0')")
+        if self.name is None or self.raw_data is None:
+            raise ValueError("A plan needs 'name' and 'raw_data'")
+        if self.dtype not in ('float32', 'float64'):
+            raise ValueError(f"Invalid dtype {self.dtype!r}")
+        if self.backend not in (None, 'numpy', 'cupy', 'jax', 'mi355x'):
+            raise ValueError(f"Unknown backend {self.backend!r}")
+        bad = set(self.init or {}) - {'state', 'scan', 'tilt', 'probe', 'object'}
+        if bad:
+            raise ValueError(f"Unknown init keys {sorted(bad)}")
+        self.init = dict(self.init or {})
+        self.slices = slice_thicknesses(self.slices)
+        self.engines = [e if isinstance(e, EnginePlan) else from_dict(_engine_entry(e)) for e in self.engines]
+
+
+def _engine_entry(e):
+    d = dict(e)
+    if isinstance(d.get('slices'), (dict, list)):
+        d['slices'] = slice_thicknesses(d['slices'])
+    if isinstance(d.get('save_options'), dict):
+        d['save_options'] = SaveOptions(**d['save_options'])
+    for k in ('shuffle_groups', 'update_probe', 'update_object', 'update_positions', 'update_tilt', 'save',
+              'save_images'):
+        if isinstance(d.get(k), dict):
+            d[k] = Flag(**d[k])
+    if isinstance(d.get('sim_shape'), list):
+        d['sim_shape'] = tuple(d['sim_shape'])
+    return d
+
+
+def plans_from_yaml(path):
+    """Every plan (YAML document) in a plan file (ReconsPlan.from_yaml_all, main.py:17)."""
+    import yaml
+    with open(path) as f:
+        docs = [d for d in yaml.safe_load_all(f) if d is not None]
+    return [ReconsPlan(**d) for d in docs]

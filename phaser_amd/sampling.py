@@ -40,6 +40,40 @@ class Sampling:
         ky, kx = self.recip_1d(dtype)
         return tuple(numpy.meshgrid(ky, kx, indexing='ij'))
 
+    def resample(self, arr, new_samp, *, order=1, cval=0.0):
+        """Resample real-space `arr` (..., ny, nx) onto `new_samp`, centres aligned
+        (num.py:712-734; linear interpolation, zeros outside)."""
+        matrix = new_samp.sampling / self.sampling
+        offset = ((self.shape - 1) - matrix * (new_samp.shape - 1)) / 2.
+        return affine_transform(arr, matrix, offset, tuple(int(v) for v in new_samp.shape), order=order, cval=cval)
+
+    def resample_recip(self, arr, new_samp, *, order=1, cval=0.0):
+        """Resample corner-origin reciprocal-space `arr` onto `new_samp`'s reciprocal grid
+        (num.py:736-754): fftshift, resample as a grid of extent k_max, ifftshift."""
+        old = Sampling(self.shape, extent=tuple(self.k_max))
+        new = Sampling(new_samp.shape, extent=tuple(new_samp.k_max))
+        arr = numpy.fft.fftshift(arr, axes=(-1, -2))
+        return numpy.fft.ifftshift(old.resample(arr, new, order=order, cval=cval), axes=(-1, -2))
+
+
+def affine_transform(arr, matrix, offset, output_shape, *, order=1, cval=0.0):
+    """scipy.ndimage.affine_transform over the last two axes of `arr`, plane by plane,
+    mode 'grid-constant' (utils/image.py:151-195); complex planes part by part."""
+    import warnings
+    import scipy.ndimage
+    arr = numpy.asarray(arr)
+    lead = arr.shape[:-2]
+    flat = arr.reshape(-1, *arr.shape[-2:])
+    out = numpy.empty((flat.shape[0], *output_shape), dtype=arr.dtype)
+    with warnings.catch_warnings():
+        # a 1-D `matrix` is the diagonal (a scale per axis), as intended
+        warnings.filterwarnings('ignore', message="The behavior of affine_transform with a 1-D array")
+        for i in range(flat.shape[0]):
+            out[i] = scipy.ndimage.affine_transform(flat[i], numpy.asarray(matrix), offset=offset,
+                                                    output_shape=output_shape, order=order, mode='grid-constant',
+                                                    cval=cval)
+    return out.reshape(*lead, *output_shape)
+
 
 class ObjectSampling:
     """Object grid (phaser/utils/object.py:189-481), the parts the engines use."""
@@ -83,6 +117,35 @@ class ObjectSampling:
         scan_min, scan_max = cls._scan_extent(scan)
         n = numpy.ceil((2. * pad + scan_max - scan_min) / sampling).astype(numpy.int_) + 1
         return cls((n[0], n[1]), sampling, scan_min - pad, scan_min, scan_max)
+
+    def expand_to_scan(self, scan, pad=0.):
+        """Grow the grid (corner moves, sampling kept) until it covers the scan plus `pad`
+        (object.py:257-275)."""
+        pad = numpy.broadcast_to(pad, (2,)).astype(numpy.float64)
+        scan_min, scan_max = self._scan_extent(numpy.asarray(scan))
+        pad_min = numpy.ceil(numpy.maximum(0, self.min - scan_min + pad) / self.sampling).astype(numpy.int_)
+        pad_max = numpy.ceil(numpy.maximum(0, scan_max - self.max + pad) / self.sampling).astype(numpy.int_)
+        if numpy.all(pad_min == 0) and numpy.all(pad_max == 0):
+            return self
+        region_min = numpy.minimum(self.region_min, scan_min) if self.region_min is not None else None
+        region_max = numpy.maximum(self.region_max, scan_max) if self.region_max is not None else None
+        return ObjectSampling(tuple(self.shape + pad_min + pad_max), self.sampling,
+                              self.corner - pad_min * self.sampling, region_min, region_max)
+
+    def with_sampling(self, sampling):
+        """The same region at another pixel size, grown to whole pixels (object.py:469-481)."""
+        sampling = numpy.broadcast_to(numpy.asarray(sampling, dtype=numpy.float64), (2,))
+        new_shape = numpy.ceil(self.sampling / sampling * self.shape).astype(int)
+        growth = sampling * new_shape - self.sampling * self.shape
+        return ObjectSampling(tuple(new_shape), sampling, self.corner - growth / 2. + (sampling - self.sampling) / 2.,
+                              self.region_min, self.region_max)
+
+    def resample(self, arr, new_samp, *, order=1, cval=1.0):
+        """Object values on `new_samp`'s grid (object.py:430-467 without rotation): linear
+        interpolation, `cval` (vacuum, 1) outside."""
+        matrix = new_samp.sampling / self.sampling
+        offset = (new_samp.corner - self.corner) / self.sampling
+        return affine_transform(arr, matrix, offset, tuple(int(v) for v in new_samp.shape), order=order, cval=cval)
 
     def check_scan(self, scan, pad=0.):
         """Raise if positions lie completely outside the object (object.py:277-289)."""
