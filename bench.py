@@ -104,10 +104,12 @@ def fft_microbench(torch, n=128, batch=2048, reps=5):
             "frac_fp32_peak": tflops / FP32_VECTOR_PEAK_TFLOPS, "gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
 
 
-def time_kernels(ws, groups, torch, n_groups):
+def time_kernels(ws, groups, torch, n_groups, constraints=()):
     """Average device duration of each kernel family over one non-graph iteration, from
     HIP events recorded on the stream the kernels are launched on (torch's current
-    stream)."""
+    stream).  `constraints` are the engine's group constraints, applied after every
+    group as in the timed iterations (without them a 24-slice object overflows and the
+    update's non-finite flagging changes its timing)."""
     from phaser_amd import _lib
     durations = {}
 
@@ -127,7 +129,7 @@ def time_kernels(ws, groups, torch, n_groups):
     try:
         for g in range(n_groups):
             ws.launch_lsqml_group(g, update_object=True, update_probe=True, update_position=False,
-                                  calc_error=False)
+                                  calc_error=False, constraints=constraints)
     finally:
         ws.call = _lib.call
         ws.concurrent = True
@@ -250,13 +252,16 @@ def main():
     state0, pats, truth = make_dataset(cfg)
     log(f"data ready in {time.perf_counter() - t0:.1f} s: patterns {pats.patterns.shape}, object {state0.object.data.shape}")
 
-    # at 24 slices from a flat start, full LSQ-ML steps (beta 1, and 0.5) overflow the
-    # object-magnitude product within the first groups, in the oracle as on the GPU; the
-    # 24-slice measurement steps with beta 0.05 (the same work per iteration)
-    from phaser_amd.plan import SOLVERS
-    beta = 0.05 if big else 1.0
-    plan = ConventionalEnginePlan(noise_model='amplitude', solver=SOLVERS['lsqml'](beta_object=beta, beta_probe=beta),
-                                  niter=args.warmup + args.steps, grouping=64)
+    # at 24 slices from a flat start, unconstrained LSQ-ML steps overflow the
+    # object-magnitude product within the first groups (in the oracle as on the GPU); the
+    # reference's remedy is the amplitude clamp as a group constraint
+    # (regularizers.py:20-42, solvers.py:113), which the 24-slice measurement applies
+    # after every group at the default beta 1
+    from phaser_amd.constraints import make_constraints
+    beta = 1.0
+    gcons = [{'type': 'clamp_object_amplitude', 'amplitude': 1.0}] if big else []
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=args.warmup + args.steps,
+                                  grouping=64, group_constraints=gcons)
     state = state0.to_device()
     groups = GroupManager(state.scan, plan.grouping, False, seed=0)
     props = make_propagators(state.probe.sampling, state.wavelength, state.object.thicknesses, plan.bwlim_frac)
@@ -268,6 +273,7 @@ def main():
     ws.set_positions(state.object.sampling, state.scan)
     sim = SimulationState(state=state, noise_model=plan.noise_model, workspace=ws)
     solver = LSQMLSolver(plan, plan.solver, use_graphs=not args.no_graphs)
+    solver.group_constraints = tuple(make_constraints(plan.group_constraints, ws, state, 'group constraint'))
     log("presolve")
     solver.presolve(sim, groups.iter(state.scan), patterns=None, pattern_mask=None, propagators=None)
     torch.cuda.synchronize()
@@ -297,7 +303,7 @@ def main():
 
     # kernel durations (HIP events on the launch stream), one extra non-graph iteration
     ws.set_groups(list(groups.iter(state.scan, 999, True)))
-    kt = time_kernels(ws, groups.iter, torch, G)
+    kt = time_kernels(ws, groups.iter, torch, G, constraints=solver.group_constraints)
     modeled = ('forward_slice', 'lsqml_backward', 'lsqml_backward_slice0', 'shift_probes', 'farfield',
                'lsqml_object_update_slices', 'lsqml_object_update_slices_begin', 'group_forward',
                'lsqml_backward_sweep')
@@ -360,9 +366,13 @@ def main():
         "config": {"workload": (f"multislice LSQ-ML: {S} slices of {ws.H}x{ws.W} complex64, {ws.n}x{ws.n} probe "
                                 f"x {M} modes, {n_pos} patterns {ws.n}x{ws.n} f32 (Poisson {cfg.dose:.0e}), groups "
                                 "of 64, 1 step = 1 iteration over all patterns"), "graphs": not args.no_graphs,
-                   "beta": beta},
+                   "beta": beta, "group_constraints": [c['type'] for c in gcons]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": None if traffic is None else (
+                         "profiles/lsqml_multislice_512_b64_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                         "per launch of this kernel, copy-calibrated, from the committed counter runs (not "
+                         "measured in this run)"),
                      "alg_bytes_per_launch": alg, "avg_launch_us": kt[dom]['avg_us'],
                      "whole_iteration_alg_GBs": whole, "whole_iteration_frac": whole / HBM_PEAK_GBS},
         "fft_rate": fft,
@@ -381,6 +391,9 @@ def main():
 
 
 class _NullObs:
+    """No per-group callbacks: the iteration runs as one captured graph."""
+    wants_groups = False
+
     def update_group(self, *a, **k):
         pass
 

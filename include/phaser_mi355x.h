@@ -130,8 +130,6 @@ typedef struct phx_group_desc {
   void* chi_slices;          /* (S-1, bmax, M, n, n) complex64 or NULL.  When set, the
                                 backward steps keep every slice's chi (chi_{S-1} stays in
                                 chi_a) so all object updates of a group can run at once */
-  void* fft_scratch;         /* (M, bmax, n, n) complex64, required for n = 192: planes larger
-                                than LDS are transposed through one plane per workgroup */
 } phx_group_desc;
 
 /* Doubles needed by phx_group_desc.red. */

@@ -33,7 +33,6 @@ class GroupDesc(ctypes.Structure):
         ("red", c_vp), ("errors", c_vp), ("aux", c_vp), ("finite_flag", c_vp),
         ("psi_slots", c_i32), ("prop_zero_waves", c_i32), ("grad_obj", c_vp), ("grad_probe", c_vp),
         ("chi_slices", c_vp),
-        ("fft_scratch", c_vp),
     ]
 
 

@@ -78,9 +78,7 @@ __global__ void __launch_bounds__(T) fft2_conv_k(const float2* __restrict__ in, 
   __shared__ float2 lds[Plane<N>::LDS_FLOAT2 + N];
   const float2* a = in + (size_t)blockIdx.x * N * N;
   float2* b = out + (size_t)blockIdx.x * N * N;
-  // a plane too large for LDS is transposed through its own output plane, which is
-  // written only after the transform (behind a barrier)
-  Plane<N> pl(Plane<N>::IN_LDS ? lds : b, lds + Plane<N>::LDS_FLOAT2);
+  Plane<N> pl(lds, lds + Plane<N>::LDS_FLOAT2);
   load_twiddles<N>(lds + Plane<N>::LDS_FLOAT2, twg);
   __syncthreads();
   const float sc = 1.0f / (float)N;
@@ -89,7 +87,6 @@ __global__ void __launch_bounds__(T) fft2_conv_k(const float2* __restrict__ in, 
     // fft2(a) = (-1)^(ky+kx) * FFT(a) / N  for even N
     pl.load_rows([&](int y, int xx) { return a[y * N + xx]; }, x);
     pl.forward_regs(x);
-    if (!Plane<N>::IN_LDS) __syncthreads();
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int ky = scr_to_freq<N>(pl.QL * k + pl.q), kx = scr_to_freq<N>(pl.rc);
@@ -105,7 +102,6 @@ __global__ void __launch_bounds__(T) fft2_conv_k(const float2* __restrict__ in, 
       x[k] = cscale(a[ky * N + kx], sg);
     }
     pl.inverse_regs(x);
-    if (!Plane<N>::IN_LDS) __syncthreads();
     pl.store_rows([&](int y, int xx, float2 v) { b[y * N + xx] = v; }, x);
   }
 }

@@ -81,19 +81,11 @@ __device__ __forceinline__ float2 shift_filter(float sy, float sx, float ky, flo
   return make_float2(cs, sn);
 }
 
-// A plane too large for LDS (192 x 192) is transposed through the workgroup's own plane
-// of d.fft_scratch (one n x n plane per workgroup of the launch grid, at most
-// n_modes x bmax).
-template <int N>
-__device__ __forceinline__ float2* plane_scratch(const phx_group_desc& d) {
-  return (float2*)d.fft_scratch + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * N * N;
-}
-
 #define PHX_PLANE_SETUP(N, T)                                                    \
   static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");               \
   __shared__ float2 lds_[Plane<N>::LDS_FLOAT2 + N];                              \
   [[maybe_unused]] __shared__ double red_lds_[T / 64];                           \
-  Plane<N> pl(Plane<N>::IN_LDS ? lds_ : plane_scratch<N>(d), lds_ + Plane<N>::LDS_FLOAT2); \
+  Plane<N> pl(lds_, lds_ + Plane<N>::LDS_FLOAT2);                                \
   load_twiddles<N>(lds_ + Plane<N>::LDS_FLOAT2, (const float2*)d.twiddle);       \
   __syncthreads();
 
@@ -1566,7 +1558,6 @@ int check_desc(const phx_group_desc* d) {
       !d->params || !d->psi || !d->red)
     return PHX_EINVAL;
   if (d->n_slices > 1 && !d->props) return PHX_EINVAL;
-  if (d->n > 128 && !d->fft_scratch) return PHX_EINVAL;  // n_modes x bmax scratch planes
   return PHX_OK;
 }
 

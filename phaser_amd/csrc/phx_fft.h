@@ -16,8 +16,10 @@
 // gives 1024 threads per plane: with one plane per CU (its LDS), that is 4 waves per
 // SIMD to hide each phase's latency instead of 2.
 //
-// A 2-D transform is: rows in registers -> transpose through LDS (at n = 192, through a
-// global scratch plane of the workgroup's own) -> columns in registers.
+// A 2-D transform is: rows in registers -> transpose through LDS -> columns in
+// registers.  At n = 192 the complex plane (288 KB) exceeds the 160 KB LDS, so the
+// transpose moves the real parts and then the imaginary parts through one 192 x 192
+// float plane (144 KB).
 // The LDS plane is padded (and for L = 8 XOR-swizzled, Plane::row_base / col_base) so that both the
 // row-phase accesses (s[r][L k + q]) and the column-phase ones (s[L j + q][c]) are
 // bank-conflict free.  A forward transform, a pointwise multiply in the (scrambled)
@@ -231,12 +233,16 @@ struct Plane {
   // conflict free.  8 lanes: no plain stride does (one direction stays 2-way), so n + 8
   // plus an XOR of column bit 2 with row bit 2 (checked exhaustively for every half-wave
   // of both access patterns: 64 distinct banks each).
-  // A plane of up to 128 x 128 is transposed through LDS; a larger one (192 x 192:
-  // 288 KB) through a global scratch plane of the workgroup's own (L2-resident), with
-  // the same barriers.
-  static constexpr bool IN_LDS = N <= 128;
-  static constexpr int LD = !IN_LDS ? N : (QL == 8 ? N + 8 : N + 4);
-  static constexpr int LDS_FLOAT2 = IN_LDS ? N * LD : 0;
+  // SPLIT (n = 192): the plane is transposed in two passes, real parts then imaginary
+  // parts, through an n x n float plane without padding.  Float element (r, c) lives at
+  // r n + (c ^ swz(r)), swz(r) = 4 b2 + 8 (b1 ^ b3) + 16 b0 + 32 b1 (b_i = bit i of r):
+  // for every wave and register, the 64 lanes of a row-phase access (16 rows x 4 lanes)
+  // and of a column-phase access (16 columns x 4 rows) hit 64 distinct banks mod 64,
+  // and each half-wave 32 distinct banks mod 32 (checked exhaustively).  swz only
+  // touches column bits 2-5, so it stays inside the aligned 64-column block.
+  static constexpr bool SPLIT = N == 192;
+  static constexpr int LD = SPLIT ? N : (QL == 8 ? N + 8 : N + 4);
+  static constexpr int LDS_FLOAT2 = SPLIT ? N * N / 2 : N * LD;
   // element (row, col) lives at row * LD + (col ^ sw(row)); for the two transpose
   // patterns the swizzle is a per-lane constant, so each is one base offset per lane
   // plus compile-time steps
@@ -246,7 +252,7 @@ struct Plane {
   __device__ __forceinline__ int col_base() const {  // (row QL j + q, col rc): + QL LD j
     return q * LD + (QL == 8 ? (rc ^ (((q >> 2) & 1) << 2)) : rc);
   }
-  float2* s;         // N x LD, in LDS or (IN_LDS false) global memory
+  float2* s;         // N x LD in LDS (SPLIT: N x N floats)
   const float2* tw;  // twiddles in LDS: tw[QL k + q] = W_N^(q k)
   int rc;            // row (row phase) or column (column phase) owned by this lane group
   int q;             // lane within the group
@@ -359,9 +365,60 @@ struct Plane {
   }
 
   // ----- transposes through LDS -----
+  // SPLIT offsets (floats).  Row phase, register k = 16 kh + kl: element (rc, 4 k + q) at
+  // rc n + q + 64 kh + (4 kl ^ swz(rc)); rc n + q has bits 2-5 clear, so with
+  // srow_base = (rc n + q) | swz(rc) it is (srow_base ^ 4 kl) + 64 kh: 16 XORs serve
+  // all 48 registers.  Column phase, register j: element (4 j + q, rc); swz(4 j + q) =
+  // (16 b0(q) + 40 b1(q)) ^ 4 (j & 3), so it is (scol_base ^ 4 (j & 3)) + 4 n j with
+  // scol_base = q n + (rc ^ (16 b0(q) + 40 b1(q))).
+  __device__ __forceinline__ static int swz(int r) {
+    return 4 * ((r >> 2) & 1) + 8 * (((r >> 1) ^ (r >> 3)) & 1) + 16 * (r & 1) + 32 * ((r >> 1) & 1);
+  }
+  __device__ __forceinline__ int srow_base() const { return (rc * N + q) | swz(rc); }
+  __device__ __forceinline__ int scol_base() const { return q * N + (rc ^ (16 * (q & 1) + 40 * (q >> 1))); }
+  __device__ __forceinline__ static int srow(int base, int k) { return (base ^ (4 * (k & 15))) + 64 * (k >> 4); }
+  __device__ __forceinline__ static int scol(int base, int j) { return (base ^ (4 * (j & 3))) + 4 * N * j; }
+  // Each pass starts from opaque copies of the bases, so the compiler computes the XORed
+  // addresses inside the pass instead of keeping 20 of them live across the transforms.
+  __device__ __forceinline__ static int opaque(int v) {
+    __asm__ volatile("" : "+v"(v));
+    return v;
+  }
+
   // row phase (x[k] at scrambled column QL k + q of row rc) -> column phase (x[j] at row
   // QL j + q of column rc)
   __device__ __forceinline__ void rows_to_cols(float2 (&x)[E]) {
+    if constexpr (SPLIT) {
+      // the column reads of a previous transpose may still be in flight: barrier first;
+      // each pass overwrites one component of x in place after all of it was written
+      float* sf = reinterpret_cast<float*>(s);
+      const int rb = srow_base(), cb = scol_base();
+      __syncthreads();
+      {
+        const int b = opaque(rb);
+#pragma unroll
+        for (int k = 0; k < E; ++k) sf[srow(b, k)] = x[k].x;
+      }
+      __syncthreads();
+      {
+        const int b = opaque(cb);
+#pragma unroll
+        for (int j = 0; j < E; ++j) x[j].x = sf[scol(b, j)];
+      }
+      __syncthreads();
+      {
+        const int b = opaque(rb);
+#pragma unroll
+        for (int k = 0; k < E; ++k) sf[srow(b, k)] = x[k].y;
+      }
+      __syncthreads();
+      {
+        const int b = opaque(cb);
+#pragma unroll
+        for (int j = 0; j < E; ++j) x[j].y = sf[scol(b, j)];
+      }
+      return;
+    }
     // The write goes to this wave's own rows.  With 8 lanes, every access of them by
     // another wave (a transpose's column reads or writes) is followed by a barrier before
     // the owner touches them again (the forward column pass's, or the one inside
@@ -377,6 +434,35 @@ struct Plane {
   // column phase (x[j] at row QL j + q of column rc) -> row phase (x[k] at column QL k + q
   // of row rc)
   __device__ __forceinline__ void cols_to_rows(float2 (&x)[E]) {
+    if constexpr (SPLIT) {
+      float* sf = reinterpret_cast<float*>(s);
+      const int rb = srow_base(), cb = scol_base();
+      __syncthreads();
+      {
+        const int b = opaque(cb);
+#pragma unroll
+        for (int j = 0; j < E; ++j) sf[scol(b, j)] = x[j].x;
+      }
+      __syncthreads();
+      {
+        const int b = opaque(rb);
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[k].x = sf[srow(b, k)];
+      }
+      __syncthreads();
+      {
+        const int b = opaque(cb);
+#pragma unroll
+        for (int j = 0; j < E; ++j) sf[scol(b, j)] = x[j].y;
+      }
+      __syncthreads();
+      {
+        const int b = opaque(rb);
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[k].y = sf[srow(b, k)];
+      }
+      return;
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < E; ++j) s[col_base() + QL * LD * j] = x[j];
@@ -513,12 +599,21 @@ struct Plane {
     } else {
       fwd<true>(x);
       __builtin_amdgcn_sched_barrier(0);
+      if constexpr (E > 16) {
+        // long lanes: eight points' multiplier loads at a time (register budget)
 #pragma unroll
-      for (int k = 0; k < E; ++k) x[k] = mul(QL * k + q, rc, x[k]);
-      // the multiplier's loads (a propagator: L2-resident) all issued before the products
-      __builtin_amdgcn_sched_group_barrier(0x0020, 2 * E, 0);
-      __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
-      __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < E; ++k) {
+          x[k] = mul(QL * k + q, rc, x[k]);
+          if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[k] = mul(QL * k + q, rc, x[k]);
+        // the multiplier's loads (a propagator: L2-resident) all issued before the products
+        __builtin_amdgcn_sched_group_barrier(0x0020, 2 * E, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0002, 1024, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
       inv<true>(x);
     }
     cols_to_rows(x);
@@ -527,8 +622,43 @@ struct Plane {
   // in-register forward -> product with the scrambled-order plane mul (conjugated with
   // CONJ) -> inverse, the multiplier's loads issued before the forward transform so
   // their latency hides behind it
+  // x[k] *= mul (conjugated with CONJ) at the column-phase points, the multiplier loaded
+  // eight points at a time: for long lanes (E = 48 at n = 192) holding all E values
+  // next to the E data points exceeds the register budget
+  template <bool CONJ>
+  __device__ __forceinline__ void mul_paced(float2 (&x)[E], const float2* mul) {
+#pragma unroll
+    for (int k0 = 0; k0 < E; k0 += 8) {
+      float2 mv[8];
+#pragma unroll
+      for (int u = 0; u < 8 && k0 + u < E; ++u) mv[u] = mul[(QL * (k0 + u) + q) * N + rc];
+#pragma unroll
+      for (int u = 0; u < 8 && k0 + u < E; ++u) x[k0 + u] = CONJ ? cmulc(x[k0 + u], mv[u]) : cmul(x[k0 + u], mv[u]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // forward -> product with the scrambled-order plane mul -> inverse, mul loaded paced
+  template <bool CONJ>
+  __device__ __forceinline__ void convolve_paced(float2 (&x)[E], const float2* mul, unsigned zero_waves = 0) {
+    fwd<false>(x);
+    rows_to_cols(x);
+    if (wave_is_zero(zero_waves)) {
+      zero_column_pass(x);
+    } else {
+      fwd<true>(x);
+      __builtin_amdgcn_sched_barrier(0);
+      mul_paced<CONJ>(x, mul);
+      inv<true>(x);
+    }
+    cols_to_rows(x);
+    inv<false>(x);
+  }
   template <bool CONJ>
   __device__ __forceinline__ void convolve_prefetched(float2 (&x)[E], const float2* mul, unsigned zero_waves = 0) {
+    if constexpr (E > 16) {
+      convolve_paced<CONJ>(x, mul, zero_waves);
+      return;
+    }
     const bool zero = wave_is_zero(zero_waves);
     float2 mv[E];
     if (!zero) {
@@ -553,6 +683,10 @@ struct Plane {
   // transpose instead: in flight during the forward column pass only
   template <bool CONJ>
   __device__ __forceinline__ void convolve_midfetch(float2 (&x)[E], const float2* mul, unsigned zero_waves = 0) {
+    if constexpr (E > 16) {
+      convolve_paced<CONJ>(x, mul, zero_waves);
+      return;
+    }
     fwd<false>(x);
     rows_to_cols(x);
     if (wave_is_zero(zero_waves)) {

@@ -24,6 +24,8 @@ from .optics import fft_lanes, scr_to_freq, twiddles
 INT32_MAX = 2**31 - 1
 # largest plane whose LSQ-ML probe step runs in strips (phx_lsqml_probe_apply_strips)
 PROBE_STRIPS_MAX_N = 128
+# largest plane side the whole-sweep kernels are used for (larger: one launch per slice)
+FUSED_MAX_N = 128
 # the object-update launch of group g also clears group g+1's accumulators
 # (phx_lsqml_object_update_slices_begin) instead of a separate phx_group_begin
 FOLD_GROUP_BEGIN = True
@@ -170,13 +172,16 @@ class GroupWorkspace:
         self.chi_slices = (torch.empty((S - 1, B, M, NN), dtype=c64, device=dev)
                            if keep_chi and S > 1 else None)
         # planes too large for LDS are transposed through one scratch plane per workgroup
-        self.fft_scratch = torch.empty((M, B, NN), dtype=c64, device=dev) if n > 128 else None
         # the work plane of the probe step in strips (n <= PROBE_STRIPS_MAX_N)
         self.probe_strips = n <= PROBE_STRIPS_MAX_N
         self.probe_tmp = torch.empty((M, NN), dtype=c64, device=dev) if self.probe_strips else None
 
-        # whole-sweep kernels (one launch per sweep) instead of one launch per slice
-        self.fused = bool(fused)
+        # whole-sweep kernels (one launch per sweep) instead of one launch per slice.  A
+        # 192 x 192 plane (48 points per lane at 768 threads, 3 waves per SIMD) leaves too
+        # few registers for a sweep that also carries its cutout and propagator loads
+        # across slices: those kernels spill, and the per-slice kernels (which do not)
+        # run a slice step 1.4-2.5x faster, so planes wider than FUSED_MAX_N use them
+        self.fused = bool(fused) and n <= FUSED_MAX_N
         # a second stream (default priority) for the probe-side kernels of a group step
         self.side_stream = torch.cuda.Stream(device=dev)
         self.concurrent = True
@@ -215,7 +220,6 @@ class GroupWorkspace:
         d.grad_obj = None if self.grad_obj is None else self.grad_obj.data_ptr()
         d.grad_probe = None if self.grad_probe is None else self.grad_probe.data_ptr()
         d.chi_slices = None if self.chi_slices is None else self.chi_slices.data_ptr()
-        d.fft_scratch = None if self.fft_scratch is None else self.fft_scratch.data_ptr()
         return d
 
     # ------------------------------------------------------------------ host tables

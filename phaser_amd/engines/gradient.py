@@ -21,7 +21,7 @@ import numpy
 
 from .. import _lib
 from ..constraints import CostRegularizers, make_constraints
-from ..device import GroupWorkspace, host_patterns, stream_handle, INT32_MAX
+from ..device import FUSED_MAX_N, GroupWorkspace, host_patterns, stream_handle, INT32_MAX
 from ..groups import GroupManager
 from ..optics import make_propagators
 from ..plan import process_flag, process_schedule, shuffle_groups_flag
@@ -65,6 +65,11 @@ class MultisliceModel:
         # with several slices, the per-slice wave-gradient planes (chi_slices)
         ok = ws.psi_slots >= ws.S and (ws.S == 1 or ws.chi_slices is not None)
         self.fused = ok if fused is None else (bool(fused) and ok)
+        if ws.n > FUSED_MAX_N:
+            self.fused = False   # per-slice kernels for wide planes (see GroupWorkspace)
+        # per-slice adjoint with every slice's wave gradient kept (chi_slices): the object
+        # gradient is gathered per tile afterwards, as in the fused path, not scattered
+        self.gather = ws.psi_slots >= ws.S and (ws.S == 1 or ws.chi_slices is not None)
 
     def dry_run(self, g):
         """errors[b] = sum(D) / model intensity of every slot (gradient/run.py:408-438)."""
@@ -138,14 +143,21 @@ class MultisliceModel:
                 _lib.call("phx_gradient_probe_finish", d, st)
             self._pending = None
             return
-        if want_object:
-            _lib.call("phx_fill_u32", _p(ws.grad_obj), 0, ws.grad_obj.numel() * 2, st)
         recompute = ws.psi_slots < ws.S
+        scatter = want_object and not self.gather
+        if scatter:
+            _lib.call("phx_fill_u32", _p(ws.grad_obj), 0, ws.grad_obj.numel() * 2, st)
         for i in range(ws.S - 1, -1, -1):
             if recompute and 1 <= i <= ws.S - 3:
                 for j in range(0, i):
                     _lib.call("phx_forward_slice", d, j, 0, st)
-            _lib.call("phx_gradient_backward", d, i, int(want_object), int(want_probe), int(want_positions), st)
+            _lib.call("phx_gradient_backward", d, i, int(scatter), int(want_probe), int(want_positions), st)
+        if want_object and self.gather:
+            if next_group is not None and next_group != g:
+                _lib.call("phx_gradient_object_gather_begin", d, 0, ws.S, ws.desc_refs[next_group], st)
+                self._begun = next_group
+            else:
+                _lib.call("phx_gradient_object_gather", d, 0, ws.S, st)
         if want_probe:
             _lib.call("phx_gradient_probe_finish", d, st)
         self._pending = None

@@ -83,6 +83,5 @@ def test_static_work_plane_rejects_sizes_beyond_the_array():
     big = _lib.GroupDesc.from_buffer_copy(ws.descs[0])
     big.n = 128
     big.n_modes = 9                      # 9 * 128^2 = 147456 > 131072
-    big.fft_scratch = None
     with pytest.raises(ValueError, match="fixed device work array"):
         _lib.call("phx_lsqml_probe_apply_strips_static", ctypes.byref(big), stream_handle())
