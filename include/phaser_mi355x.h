@@ -402,10 +402,11 @@ int phx_f32_to_bf16(const float* in, void* out, int64_t n, hipStream_t stream);
  * target)^2 in FP64 when loss != NULL. */
 int phx_mse_grad(const float* pred, const float* target, int64_t n, float scale, float* d, void* db, double* loss,
                  hipStream_t stream);
-/* Network input: out[r] = log(1 + bin2x2(pattern[index[r]]) * inv_scale), (n/2)^2 BF16
- * values per row; patterns (P, n, n) FP32 corner-origin; index NULL means r. */
-int phx_pattern_features(const float* patterns, const int32_t* index, int64_t count, int64_t n, float inv_scale,
-                         void* out, hipStream_t stream);
+/* Network input: out[r] = log(1 + binned(pattern[index[r]]) * inv_scale), the sums of
+ * bin x bin pixel blocks, (n/bin)^2 BF16 values per row (n a multiple of bin); patterns
+ * (P, n, n) FP32 corner-origin; index NULL means r. */
+int phx_pattern_features(const float* patterns, const int32_t* index, int64_t count, int64_t n, int64_t bin,
+                         float inv_scale, void* out, hipStream_t stream);
 /* Training targets: out[r] = [phase - phase0, amplitude] of prod_s O_s over the ps x ps
  * window starting at starts[r] = (y0, x0) (2 ps^2 FP32 per row). */
 int phx_object_patches(const void* obj, int64_t S, int64_t H, int64_t W, const int32_t* starts, int64_t count,

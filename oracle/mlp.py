@@ -73,11 +73,11 @@ def adam(w, g, m, v, t, *, lr, b1=0.9, b2=0.999, eps=1e-8, weight_decay=0.0):
     return w
 
 
-def features(patterns, inv_scale):
-    """log(1 + 2x2-binned pattern * inv_scale), flattened rows."""
+def features(patterns, inv_scale, bin=2):
+    """log(1 + (bin x bin)-binned pattern * inv_scale), flattened rows."""
     P = numpy.asarray(patterns, dtype=numpy.float64)
     n = P.shape[-1]
-    b = P.reshape(*P.shape[:-2], n // 2, 2, n // 2, 2).sum(axis=(-3, -1))
+    b = P.reshape(*P.shape[:-2], n // bin, bin, n // bin, bin).sum(axis=(-3, -1))
     return numpy.log1p(numpy.maximum(b, 0.0) * inv_scale).reshape(*P.shape[:-2], -1)
 
 

@@ -82,7 +82,7 @@ SIGNATURES = {
                                   c_f32, c_vp]),
     "phx_f32_to_bf16": (c_int, [c_vp, c_vp, c_i64, c_vp]),
     "phx_mse_grad": (c_int, [c_vp, c_vp, c_i64, c_f32, c_vp, c_vp, c_vp, c_vp]),
-    "phx_pattern_features": (c_int, [c_vp, c_vp, c_i64, c_i64, c_f32, c_vp, c_vp]),
+    "phx_pattern_features": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_f32, c_vp, c_vp]),
     "phx_object_patches": (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_f32, c_vp, c_vp]),
     "phx_stitch_patches": (c_int, [c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp,
                                    c_vp]),

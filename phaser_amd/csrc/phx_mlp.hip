@@ -166,16 +166,17 @@ __global__ void mse_grad_k(const float* __restrict__ pred, const float* __restri
 
 // Features: 2 x 2 binned corner-origin pattern, log(1 + I / scale), as BF16 rows.
 __global__ void pattern_features_k(const float* __restrict__ pats, const int32_t* __restrict__ index, int64_t count,
-                                   int n, float inv_scale, __bf16* __restrict__ out) {
-  const int h = n / 2, hh = h * h;
+                                   int n, int bin, float inv_scale, __bf16* __restrict__ out) {
+  const int h = n / bin, hh = h * h;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count * hh) return;
   const int64_t r = t / hh;
   const int e = (int)(t % hh), y = e / h, x = e % h;
   const int64_t p = index ? index[r] : r;
-  const float* P = pats + p * (int64_t)n * n;
-  const float s = P[(2 * y) * n + 2 * x] + P[(2 * y) * n + 2 * x + 1] + P[(2 * y + 1) * n + 2 * x] +
-                  P[(2 * y + 1) * n + 2 * x + 1];
+  const float* P = pats + p * (int64_t)n * n + (int64_t)(bin * y) * n + bin * x;
+  float s = 0.f;  // the bin x bin block summed row by row
+  for (int i = 0; i < bin; ++i)
+    for (int j = 0; j < bin; ++j) s += P[i * n + j];
   out[r * hh + e] = to_bf16(log1pf(fmaxf(s, 0.f) * inv_scale));
 }
 
@@ -294,12 +295,12 @@ int phx_mse_grad(const float* pred, const float* target, int64_t n, float scale,
   return PHX_OK;
 }
 
-int phx_pattern_features(const float* patterns, const int32_t* index, int64_t count, int64_t n, float inv_scale,
-                         void* out, hipStream_t stream) {
-  if (!patterns || !out || count <= 0 || n < 2 || (n & 1)) return PHX_EINVAL;
-  const int64_t total = count * (n / 2) * (n / 2);
+int phx_pattern_features(const float* patterns, const int32_t* index, int64_t count, int64_t n, int64_t bin,
+                         float inv_scale, void* out, hipStream_t stream) {
+  if (!patterns || !out || count <= 0 || bin < 1 || n < bin || n % bin) return PHX_EINVAL;
+  const int64_t total = count * (n / bin) * (n / bin);
   hipLaunchKernelGGL(pattern_features_k, dim3(blocks_for(total, 256)), dim3(256), 0, stream, patterns, index, count,
-                     (int)n, inv_scale, (__bf16*)out);
+                     (int)n, (int)bin, inv_scale, (__bf16*)out);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

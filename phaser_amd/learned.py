@@ -169,12 +169,20 @@ class MLP:
 
 # ----------------------------------------------------------------------------- data
 
-def features(patterns_dev, n, index=None, inv_scale=1.0):
-    """(count, n^2/4) BF16 network inputs from device patterns (P, n, n) FP32."""
+def feature_bin(n):
+    """Pattern binning that gives the network's 64 x 64 input (2 at 128^2, 3 at 192^2)."""
+    if n % 64:
+        raise ValueError(f"pattern side {n} is not a multiple of 64")
+    return n // 64
+
+
+def features(patterns_dev, n, index=None, inv_scale=1.0, bin=2):
+    """(count, (n/bin)^2) BF16 network inputs from device patterns (P, n, n) FP32."""
     torch = _torch()
     count = patterns_dev.shape[0] if index is None else index.shape[0]
-    out = torch.empty((count, (n // 2)**2), dtype=torch.bfloat16, device=patterns_dev.device)
-    _lib.call("phx_pattern_features", _p(patterns_dev), _p(index), count, n, float(inv_scale), _p(out), _stream())
+    out = torch.empty((count, (n // bin)**2), dtype=torch.bfloat16, device=patterns_dev.device)
+    _lib.call("phx_pattern_features", _p(patterns_dev), _p(index), count, n, bin, float(inv_scale), _p(out),
+              _stream())
     return out
 
 
@@ -229,6 +237,21 @@ def nrmse(obj, truth, crop=None):
     return float(numpy.linalg.norm(a * numpy.exp(1j * phi) - b) / numpy.linalg.norm(b))
 
 
+def nrmse_tensor(obj, truth, crop=None):
+    """nrmse() on torch tensors where they live (the GPU for a resident object), in
+    complex128: no host copy of a large multislice object."""
+    import torch
+    sel = (slice(None),) + (tuple(crop) if crop is not None else ())
+    a, b = obj[sel], truth[sel]
+    pa = a[0].to(torch.complex128)
+    pb = b[0].to(torch.complex128)
+    for s in range(1, a.shape[0]):
+        pa = pa * a[s]
+        pb = pb * b[s]
+    phi = torch.angle(torch.sum(torch.conj(pa) * pb))
+    return float(torch.linalg.vector_norm(pa * torch.exp(1j * phi) - pb) / torch.linalg.vector_norm(pb))
+
+
 def reconstruct_plain(state, pats, *, niter, grouping=64, seed=0):
     """Plain LSQ-ML (the engines' defaults) from the given state; returns the result."""
     from .engines.conventional import run_engine
@@ -238,17 +261,37 @@ def reconstruct_plain(state, pats, *, niter, grouping=64, seed=0):
     return run_engine(args, plan)
 
 
-def training_examples(cfg, *, niter, log=None):
-    """One synthetic object reconstructed by plain LSQ-ML: (features BF16, targets FP32)
-    on the device, targets taken from the normalized reconstruction."""
+def reconstruct_gradient(state, pats, *, niter, grouping=64, seed=0, noise_model='poisson', lr_object=20.0,
+                         momentum=0.5, group_constraints=(), observer=None):
+    """The gradient engine (hand-written adjoint) with momentum SGD on the object only,
+    from the given state; returns the result."""
+    from .engines import gradient as geng
+    from .plan import GradientEnginePlan
+    plan = GradientEnginePlan(noise_model=noise_model, niter=niter, grouping=grouping,
+                              solvers={'object': {'type': 'sgd', 'learning_rate': lr_object, 'momentum': momentum}},
+                              group_constraints=list(group_constraints))
+    args = {'data': pats, 'state': state.to_device(), 'dtype': numpy.float32, 'seed': seed, 'observer': observer,
+            'recons_name': ''}
+    return geng.run_engine(args, plan)
+
+
+def training_examples(cfg, *, niter, method='lsqml', gradient_opts=None, log=None):
+    """One synthetic object reconstructed by the method (plain LSQ-ML, or with
+    method='gradient' the gradient engine with `gradient_opts`): (features BF16, targets
+    FP32) on the device, targets taken from the normalized reconstruction."""
     torch = _torch()
     from .synthetic import make_dataset
     state, pats, _truth = make_dataset(cfg)
-    out = reconstruct_plain(state, pats, niter=niter, seed=cfg.seed).to_numpy()
+    if method == 'lsqml':
+        out = reconstruct_plain(state, pats, niter=niter, seed=cfg.seed).to_numpy()
+    elif method == 'gradient':
+        out = reconstruct_gradient(state, pats, niter=niter, seed=cfg.seed, **(gradient_opts or {})).to_numpy()
+    else:
+        raise ValueError(f"unknown reconstruction method '{method}'")
     obj = normalize_object(out.object.data)
     n = cfg.n
     dev_p = torch.from_numpy(numpy.ascontiguousarray(pats.patterns.reshape(-1, n, n))).cuda()
-    X = features(dev_p, n)
+    X = features(dev_p, n, bin=feature_bin(n))
     starts = torch.from_numpy(window_starts(out.object.sampling, out.scan, n)).cuda()
     T = patch_targets(torch.from_numpy(obj.astype(numpy.complex64)).cuda(), starts)
     return X, T
@@ -291,9 +334,11 @@ def train(mlp, X, T, *, epochs, batch=1024, lr=1e-4, seed=0, log=None, standardi
     return hist
 
 
-def learned_object(mlp, pats, state, ps=PATCH):
+def learned_object(mlp, pats, state, ps=PATCH, phase_only=False):
     """The stitched starting object (S, H, W) complex64 on the device for `state`'s scan
-    and object grid, from the network's guesses on every pattern."""
+    and object grid, from the network's guesses on every pattern.  With `phase_only` the
+    predicted amplitudes are replaced by 1 (a phase object: the guess carries only the
+    network's patch phases)."""
     torch = _torch()
     n = int(state.probe.data.shape[-1])
     S, H, W = (int(v) for v in state.object.data.shape)
@@ -301,8 +346,10 @@ def learned_object(mlp, pats, state, ps=PATCH):
         dev_p = pats.patterns.reshape(-1, n, n)
     else:
         dev_p = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(pats.patterns).reshape(-1, n, n))).cuda()
-    X = features(dev_p, n)
+    X = features(dev_p, n, bin=feature_bin(n))
     pred = mlp.predict(X)
+    if phase_only:
+        pred[:, ps * ps:] = 1.0
     starts = torch.from_numpy(window_starts(state.object.sampling, state.scan, n, ps)).cuda()
     return stitch(pred, starts, S, H, W, ps)
 
@@ -319,7 +366,9 @@ def evaluation_config(seed=7):
 
 class NrmseWatch:
     """Observer with the engines' callback names: after every iteration, the object's
-    NRMSE against `truth` over `crop`; raises EarlyTermination at `threshold`."""
+    NRMSE against `truth` over `crop`; raises EarlyTermination at `threshold`.  It needs
+    no per-group callbacks, so the engines run their captured-graph iterations."""
+    wants_groups = False
 
     def __init__(self, truth, crop, threshold, t0):
         self.truth, self.crop, self.threshold, self.t0 = truth, crop, threshold, t0
@@ -342,7 +391,12 @@ class NrmseWatch:
         torch = _torch()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        e = nrmse(obj.detach().cpu().numpy() if hasattr(obj, 'detach') else obj, self.truth, self.crop)
+        if hasattr(obj, 'is_cuda') and obj.is_cuda:
+            if not hasattr(self, 'truth_dev'):
+                self.truth_dev = torch.from_numpy(numpy.ascontiguousarray(self.truth)).to(obj.device)
+            e = nrmse_tensor(obj.detach(), self.truth_dev, self.crop)
+        else:
+            e = nrmse(obj, self.truth, self.crop)
         self.history.append((t - self.t0 - self.paused, it, e))
         self.paused += time.perf_counter() - t
         return e
@@ -354,19 +408,26 @@ class NrmseWatch:
 
 
 def _watch_run(run, watch):
+    """Run one arm under `watch`; an engine's non-finite-state error (the reference's
+    ValueError naming the first bad group) ends the arm and is recorded as `diverged`."""
     from .state import EarlyTermination
+    diverged = None
     try:
         run(watch)
     except EarlyTermination:
         pass
+    except ValueError as e:
+        if 'NaN or inf' not in str(e):
+            raise
+        diverged = str(e)
     hit = [h for h in watch.history if h[2] <= watch.threshold]
     return {'seconds': hit[0][0] if hit else None, 'iterations': hit[0][1] if hit else None,
-            'final_nrmse': watch.history[-1][2] if watch.history else None,
+            'final_nrmse': watch.history[-1][2] if watch.history else None, 'diverged': diverged,
             'history': [(round(t, 4), i, round(e, 5)) for t, i, e in watch.history]}
 
 
 def time_to_nrmse(cfg, mlp, *, threshold=0.05, max_iter=60, grouping=64, lr_object=10.0, momentum=0.9, log=None,
-                  data=None):
+                  data=None, gradient_opts=None, plain_plan=None, phase_only_init=False):
     """Wall time to object NRMSE <= threshold against the synthetic truth, on the problem
     `cfg` (patterns resident before the clock starts):
       plain:   LSQ-ML from the flat start;
@@ -375,10 +436,13 @@ def time_to_nrmse(cfg, mlp, *, threshold=0.05, max_iter=60, grouping=64, lr_obje
                object: per-pixel normalised steps (Adam) amplify the weakly illuminated
                edge and stall near NRMSE 0.065 on these objects);
       flat:    the same gradient engine from the flat start (what the network adds).
-    NRMSE: projected object prod_s O_s over the scanned region, optimal global phase."""
+    NRMSE: projected object prod_s O_s over the scanned region, optimal global phase.
+    `gradient_opts` (reconstruct_gradient's keywords) replaces the gradient arms' default
+    recipe (amplitude noise model, SGD lr_object / momentum); `plain_plan(niter)` gives the
+    plain arm's ConventionalEnginePlan (default: LSQ-ML with the engine defaults);
+    `phase_only_init` starts the learned arm from the network's phases alone (learned_object).  An arm whose
+    state goes non-finite is recorded as diverged."""
     torch = _torch()
-    from .engines import gradient as geng
-    from .plan import GradientEnginePlan
     from .synthetic import make_dataset
     state0, pats, truth = data if data is not None else make_dataset(cfg)
     # the patterns live in HBM before any clock starts (every run reads them there)
@@ -390,39 +454,48 @@ def time_to_nrmse(cfg, mlp, *, threshold=0.05, max_iter=60, grouping=64, lr_obje
         'object': list(state0.object.data.shape), 'patterns': int(numpy.prod(cfg.scan_shape))}}
     res['flat_start_nrmse'] = nrmse(state0.object.data, truth, crop)
 
-    def gradient_run(st, watch, niter):
-        p = GradientEnginePlan(noise_model='amplitude', niter=niter, grouping=grouping,
-                               solvers={'object': {'type': 'sgd', 'learning_rate': lr_object,
-                                                   'momentum': momentum}})
-        geng.run_engine({'data': pats, 'state': st, 'dtype': numpy.float32, 'seed': 0, 'observer': watch,
-                         'recons_name': ''}, p)
+    gopts = gradient_opts if gradient_opts is not None else {
+        'noise_model': 'amplitude', 'lr_object': lr_object, 'momentum': momentum}
+    res['gradient_recipe'] = {k: (list(v) if isinstance(v, tuple) else v) for k, v in gopts.items()}
 
-    # warm every path once (kernel loading, first-use costs) outside the clock
-    reconstruct_plain(state0.copy(), pats, niter=1)
+    def gradient_run(st, watch, niter):
+        reconstruct_gradient(st, pats, niter=niter, grouping=grouping, seed=0, observer=watch, **gopts)
+
+    def make_plain_plan(niter):
+        from .plan import ConventionalEnginePlan
+        if plain_plan is not None:
+            return plain_plan(niter)
+        return ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=niter, grouping=grouping)
+
+    def plain_run(st, watch, niter):
+        from .engines.conventional import run_engine
+        run_engine({'data': pats, 'state': st, 'dtype': numpy.float32, 'seed': 0, 'observer': watch,
+                    'recons_name': ''}, make_plain_plan(niter))
+
+    # warm every path once (kernel loading, first-use costs) outside the clock; a plain
+    # recipe that diverges on this problem does so here too
+    try:
+        plain_run(state0.copy(), None, 1)
+    except ValueError as e:
+        if 'NaN or inf' not in str(e):
+            raise
     gradient_run(state0.copy().to_device(), None, 1)
-    learned_object(mlp, pats, state0)
+    learned_object(mlp, pats, state0, phase_only=phase_only_init)
     torch.cuda.synchronize()
 
     t0 = time.perf_counter()
     w = NrmseWatch(truth, crop, threshold, t0)
-    plan = None
-
-    def plain(watch):
-        from .engines.conventional import run_engine
-        from .plan import ConventionalEnginePlan
-        p = ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=max_iter, grouping=grouping)
-        run_engine({'data': pats, 'state': state0.copy(), 'dtype': numpy.float32, 'seed': 0, 'observer': watch,
-                    'recons_name': ''}, p)
-    res['plain_lsqml'] = _watch_run(plain, w)
+    res['plain_lsqml'] = _watch_run(lambda watch: plain_run(state0.copy(), watch, max_iter), w)
     if log:
-        log(f"plain LSQ-ML: {res['plain_lsqml']['seconds']} s, final NRMSE {res['plain_lsqml']['final_nrmse']}")
+        log(f"plain LSQ-ML: {res['plain_lsqml']['seconds']} s, final NRMSE {res['plain_lsqml']['final_nrmse']}"
+            + (f", diverged: {res['plain_lsqml']['diverged']}" if res['plain_lsqml']['diverged'] else ""))
 
     t0 = time.perf_counter()
     w = NrmseWatch(truth, crop, threshold, t0)
 
     def learned(watch):
         st = state0.copy().to_device()
-        st.object.data = learned_object(mlp, pats, st).contiguous()
+        st.object.data = learned_object(mlp, pats, st, phase_only=phase_only_init).contiguous()
         watch.check(st.object.data, 0)
         res['learned_init_nrmse'] = watch.history[-1][2]
         gradient_run(st, watch, max_iter)
@@ -443,9 +516,11 @@ def time_to_nrmse(cfg, mlp, *, threshold=0.05, max_iter=60, grouping=64, lr_obje
 
 
 def train_on_reconstructions(n_objects, *, base=None, niter=10, epochs=20, lr=1e-4, batch=1024, seed=0,
-                             sizes=DEFAULT_SIZES, log=None):
-    """Build the training set from plain-LSQ-ML reconstructions of `n_objects` seeded
-    synthetic objects (3-5 slices), then train a fresh network.  Returns (mlp, info)."""
+                             sizes=DEFAULT_SIZES, method='lsqml', gradient_opts=None, vary_slices=True, log=None):
+    """Build the training set from reconstructions (plain LSQ-ML, or the gradient engine
+    with method='gradient') of `n_objects` seeded synthetic objects (3-5 slices, or the
+    base configuration's slice count without `vary_slices`), then train a fresh network.
+    Returns (mlp, info)."""
     torch = _torch()
     from dataclasses import replace
     from .synthetic import SyntheticConfig
@@ -454,8 +529,8 @@ def train_on_reconstructions(n_objects, *, base=None, niter=10, epochs=20, lr=1e
     t0 = time.perf_counter()
     Xs, Ts = [], []
     for k in range(n_objects):
-        cfg = replace(base, n_slices=3 + k % 3, seed=seed * 100003 + 1000 + k)
-        X, T = training_examples(cfg, niter=niter)
+        cfg = replace(base, n_slices=(3 + k % 3) if vary_slices else base.n_slices, seed=seed * 100003 + 1000 + k)
+        X, T = training_examples(cfg, niter=niter, method=method, gradient_opts=gradient_opts)
         Xs.append(X)
         Ts.append(T)
     X = torch.cat(Xs)
@@ -470,4 +545,5 @@ def train_on_reconstructions(n_objects, *, base=None, niter=10, epochs=20, lr=1e
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     return mlp, {'rows': int(X.shape[0]), 'objects': n_objects, 'data_seconds': t_data, 'train_seconds': t_train,
-                 'epochs': epochs, 'loss_first_last': [hist[0], hist[-1]]}
+                 'epochs': epochs, 'loss_first_last': [hist[0], hist[-1]], 'method': method,
+                 'reconstruction_iterations': niter}

@@ -9,7 +9,8 @@ scans small enough for the numpy oracle (seconds), plus full-size property check
     (a momentum SGD solver on the positions) against the oracle's gradient engine;
   * config 5: 192^2 x 8 modes, 24 slices, LSQ-ML on a 2 x 2 scan;
   * full size, configs 2 and 3: one iteration of the whole problem is finite and lowers
-    the detector error, and group 0's step equals the oracle's on that group.
+    the detector error, and group 0's step equals the oracle's on that group;
+  * full size, config 5: groups 0 and 1 of the 24-slice problem step as the oracle's.
 """
 import numpy
 import pytest
@@ -233,3 +234,62 @@ def test_config3_full_size_iteration_and_first_group():
           f"positions {e[3]:.2e}", flush=True)
     assert numpy.all(g_obj[~covered] == 0)
     assert e[0] < 1e-4 and e[1] < 1e-4 and e[2] < 1e-4 and e[3] < 1e-3
+
+
+def test_config5_full_size_first_groups():
+    """The full 24-slice problem (1540^2 x 24 slices, 192^2 x 8 modes, 36,864 positions):
+    from the GPU presolve's probe, LSQ-ML groups 0 and 1 of iteration 1 (reference
+    defaults, beta 1; object steps only) change the object as the oracle's lsqml_run does
+    on those groups in sequence.  The steps are large (the per-slice step sizes add up over
+    24 slices: after group 1 the object amplitude spans about 0.01-6), which makes the
+    comparison a strong one; the values stay finite."""
+    import torch
+    import oracle.solvers as osol
+    from oracle.optics import make_propagators as omake_props
+    from phaser_amd.device import GroupWorkspace
+    from phaser_amd.engines.conventional import LSQMLSolver, SimulationState
+    from phaser_amd.groups import GroupManager
+    from phaser_amd.optics import make_propagators
+    from phaser_amd.plan import ConventionalEnginePlan
+    from phaser_amd.synthetic import LSQML_192X8_S24
+    state, pats, _truth = _full(LSQML_192X8_S24)
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=1, grouping=64)
+    dev = state.copy().to_device()
+    props = make_propagators(dev.probe.sampling, dev.wavelength, dev.object.thicknesses, 2 / 3)
+    groups = GroupManager(dev.scan, 64, False, seed=7)
+    ws = GroupWorkspace(dev, pats.patterns, pats.pattern_mask, props, noise_offset=0.0, noise_eps=1e-3,
+                        n_groups=len(groups), bmax=64, keep_chi=True)
+    ws.set_positions(dev.object.sampling, dev.scan)
+    solver = LSQMLSolver(plan, plan.solver, use_graphs=False)
+    sim = SimulationState(state=dev, noise_model=plan.noise_model, workspace=ws)
+    solver.presolve(sim, groups.iter(dev.scan), patterns=pats.patterns, pattern_mask=pats.pattern_mask,
+                    propagators=props)
+    obj0, probe0 = ws.object.cpu().numpy(), ws.probe.cpu().numpy()
+    glist = list(groups.iter(dev.scan, 1, True))
+    ws.set_groups(glist)
+    solver._set_params(sim)
+    ws.reset_finite_flag()
+
+    o = to_oracle_state(state)
+    o.probe.data = probe0.copy()
+    osim = osol.Sim(o, onoise.AmplitudeNoiseModel(), numpy.float32)
+    ky, kx = o.probe.sampling.recip_grid(dtype=numpy.float32)
+    oprops = omake_props(ky, kx, o.probe.sampling.k_max, o.wavelength, o.object.thicknesses, 2 / 3)
+    for g in range(2):
+        print(f"config 5 full: group {g}", flush=True)
+        ws.launch_lsqml_group(g, update_object=True, update_probe=False, update_position=False, calc_error=True)
+        torch.cuda.synchronize()
+        gobj = ws.object.cpu().numpy()
+        g_err = ws.errors[g, :64].cpu().numpy()
+        grp = glist[g]
+        errors, _ = osol.lsqml_run(osim, grp, pats.patterns[tuple(grp)], pattern_mask=pats.pattern_mask,
+                                   props=oprops, beta_object=1.0, beta_probe=1.0, update_object=True,
+                                   update_probe=False, update_position=False, calc_error=True,
+                                   illum_reg_object=1e-2, illum_reg_probe=1e-2, gamma=1e-4)
+        eo = rel_err(gobj - obj0, o.object.data - obj0)
+        ee = rel_err(g_err, numpy.asarray(errors).reshape(-1))
+        amp = numpy.abs(o.object.data)
+        print(f"config 5 full, group {g}: object update {eo:.2e} errors {ee:.2e}; oracle |O| in "
+              f"[{amp.min():.3g}, {amp.max():.3g}]", flush=True)
+        assert numpy.all(numpy.isfinite(gobj))
+        assert eo < 1e-4 and ee < 1e-5

@@ -104,6 +104,11 @@ def test_features_patches_and_stitch():
     f = features(torch.from_numpy(pats).cuda(), n, inv_scale=0.5).float().cpu().numpy()
     ref = om.round_bf16(om.features(pats, 0.5))
     assert numpy.max(numpy.abs(f - ref) / numpy.abs(ref)) <= 2**-7   # at most one bf16 ulp
+    # 3 x 3 binning (192^2 patterns -> 64^2 inputs) on a 24 x 24 pattern
+    p24 = rng.uniform(0, 50, size=(3, 24, 24)).astype(numpy.float32)
+    f3 = features(torch.from_numpy(p24).cuda(), 24, inv_scale=0.5, bin=3).float().cpu().numpy()
+    ref3 = om.round_bf16(om.features(p24, 0.5, bin=3))
+    assert f3.shape == (3, 64) and numpy.max(numpy.abs(f3 - ref3) / numpy.abs(ref3)) <= 2**-7
     # windows tiling a 2-slice object: extracting then stitching returns the projected
     # object's phase split equally between the slices
     S, H, W, ps = 2, 40, 48, 8

@@ -93,6 +93,12 @@ def test_features_closed_form():
     f = om.features(p, 0.5)
     bins = numpy.array([[0 + 1 + 4 + 5, 2 + 3 + 6 + 7], [8 + 9 + 12 + 13, 10 + 11 + 14 + 15]])
     numpy.testing.assert_allclose(f, numpy.log1p(bins * 0.5).ravel())
+    # 3 x 3 bins of a 6 x 6 pattern: block (i, j) sums 6 (3 i + a) + 3 j + b over a, b < 3
+    p6 = numpy.arange(36, dtype=numpy.float64).reshape(6, 6)
+    f3 = om.features(p6, 1.0, bin=3)
+    blocks = [sum(6 * (3 * i + a) + 3 * j + b for a in range(3) for b in range(3)) for i in range(2) for j in range(2)]
+    assert blocks == [63, 90, 225, 252]
+    numpy.testing.assert_allclose(f3, numpy.log1p(numpy.array(blocks, dtype=numpy.float64)))
 
 
 def test_nrmse_phase_invariance_and_value():
@@ -104,3 +110,17 @@ def test_nrmse_phase_invariance_and_value():
     assert om.nrmse(t2, t) == pytest.approx(0.1, rel=1e-12)
     crop = (slice(2, 8), slice(1, 9))
     assert om.nrmse(numpy.ones_like(t), t, crop) > 0
+
+
+def test_tensor_nrmse_matches_the_oracle():
+    """The product-path NRMSE on torch tensors (used where the object is resident on the
+    device) equals the oracle's on the same arrays, with and without a crop."""
+    import torch
+    from phaser_amd.learned import nrmse_tensor
+    rng = numpy.random.default_rng(11)
+    t = numpy.exp(1j * rng.uniform(-0.3, 0.3, (3, 12, 14))).astype(numpy.complex64)
+    o = (t * numpy.exp(1j * rng.normal(0, 0.05, t.shape)) * rng.uniform(0.95, 1.05, t.shape)).astype(numpy.complex64)
+    crop = (slice(2, 10), slice(3, 12))
+    for c in (None, crop):
+        assert nrmse_tensor(torch.from_numpy(o), torch.from_numpy(t), c) == pytest.approx(om.nrmse(o, t, c), rel=1e-12)
+    assert nrmse_tensor(torch.from_numpy(t * numpy.exp(0.4j).astype(numpy.complex64)), torch.from_numpy(t)) < 1e-6
