@@ -25,7 +25,7 @@ from ..device import FUSED_MAX_N, GroupWorkspace, host_patterns, stream_handle, 
 from ..groups import GroupManager
 from ..optics import make_propagators
 from ..plan import process_flag, process_schedule, shuffle_groups_flag
-from .conventional import Observer
+from .conventional import Observer, check_no_tilt
 
 logger = logging.getLogger(__name__)
 
@@ -353,6 +353,7 @@ def run_engine(args, props, *, recompute=False):
     seed = args.get('seed')
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
+    check_no_tilt(state)
     noise = props.noise_model
     all_vars, group_solvers, iter_solvers = process_solvers(props)
     if 'tilt' in all_vars:

@@ -3,6 +3,7 @@ layout (phaser/state.py:17-211); phaser itself is not importable here."""
 from types import SimpleNamespace as NS
 
 import numpy
+import pytest
 
 from phaser_amd.interop import patterns_from_phaser, plan_from_phaser, state_from_phaser, state_into_phaser
 from phaser_amd.plan import ConventionalEnginePlan, GradientEnginePlan
@@ -216,3 +217,16 @@ def test_position_solvers_take_the_reference_protocol():
     numpy.testing.assert_allclose(u, 1.5)
     sd = make_position_solver(SteepestDescentPositionSolverPlan(step_size=0.1))
     assert sd.init_state(sim) is None and sd.max_step_size == 0.1   # position_correction.py:16
+
+
+def test_engines_refuse_per_position_tilts():
+    """A state with non-zero tilts needs tilted propagators, which the kernels do not
+    take: both engines refuse it before touching the GPU (all-zero tilts are accepted)."""
+    from phaser_amd.engines.conventional import check_no_tilt
+    st = NS(tilt=None)
+    check_no_tilt(st)
+    st.tilt = numpy.zeros((3, 3, 2), dtype=numpy.float32)
+    check_no_tilt(st)
+    st.tilt[1, 1, 0] = 1e-3
+    with pytest.raises(NotImplementedError, match="tilts"):
+        check_no_tilt(st)
