@@ -130,7 +130,24 @@ typedef struct phx_group_desc {
   void* chi_slices;          /* (S-1, bmax, M, n, n) complex64 or NULL.  When set, the
                                 backward steps keep every slice's chi (chi_{S-1} stays in
                                 chi_a) so all object updates of a group can run at once */
+  const float* tilt;         /* (P, 2) float32 tangents (tan(1e-3 tilt_y), tan(1e-3 tilt_x)) of
+                                each position's tilt in mrad, or NULL for no tilt.  Propagator
+                                j of position p is then props[j] exp(2 pi i dz_j (ty ky + tx kx))
+                                (tilt_propagators, phaser/engines/common/simulation.py:193-222),
+                                and every kernel that propagates reads the group's tilted copies
+                                in tprops, which phx_tilt_propagators fills */
+  const float* slice_dz;     /* (S-1) float32 slice thicknesses dz_j (length units); read only
+                                when tilt is set */
+  void* tprops;              /* (S-1, bmax, n*n) complex64, scrambled: the group's tilted
+                                propagators (used only when tilt is set) */
 } phx_group_desc;
+
+/* Tilted propagators of the group's slots (tilt_propagators, phaser/engines/common/
+ * simulation.py:193-222): tprops[j][b] = props[j] exp(2 pi i dz_j (ty ky + tx kx)) with the
+ * tangents of slot b's position, the phase taken in half-turns (sincospif).  Launched
+ * before a group's forward model (and again before its adjoint if another group was
+ * filled in between) when d->tilt is set; PHX_EINVAL when it is not. */
+int phx_tilt_propagators(const phx_group_desc* d, hipStream_t stream);
 
 /* Doubles needed by phx_group_desc.red. */
 int64_t phx_red_doubles(int32_t n_slices, int32_t bmax, int32_t n_modes);

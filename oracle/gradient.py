@@ -14,7 +14,7 @@ import numpy
 
 from .misc import GroupManager
 from .num import fft2, ifft2, abs2, check_finite, to_complex_dtype
-from .optics import fourier_shift_filter, make_propagators
+from .optics import fourier_shift_filter, make_propagators, tilt_propagators
 
 
 def _cut(state, group, n):
@@ -35,6 +35,15 @@ def loss_from_intensity(noise, model_intensity, patterns, mask):
     return 2. * numpy.sum(mask * (numpy.sqrt(p + c) - numpy.sqrt(model_intensity + c) - eps)**2) / var
 
 
+def group_props(state, group, ky, kx, props):
+    """The group's propagators, tilted per position when the state has tilts
+    (run.py:373,391: tilt_propagators of the group's tilts): (S-1, B or 1, n, n)."""
+    if props is None:
+        return None
+    tilts = state.tilt[tuple(group)] if state.tilt is not None else None
+    return tilt_propagators(ky, kx, state.object.thicknesses, props, tilts)
+
+
 def forward(state, group, ky, kx, props):
     """Shifted probes, per-slice waves, exit spectra (run.py:375-393)."""
     probe = state.probe.data
@@ -44,8 +53,9 @@ def forward(state, group, ky, kx, props):
     psi = ifft2(fft2(probe) * filt)
     waves = [psi]
     S = group_obj.shape[1]
+    tprops = group_props(state, group, ky, kx, props)
     for s in range(S - 1):
-        psi = ifft2(fft2(psi * group_obj[:, s, None]) * props[s][None, None])
+        psi = ifft2(fft2(psi * group_obj[:, s, None]) * tprops[s][:, None])
         waves.append(psi)
     model_wave = fft2(psi * group_obj[:, -1, None])
     return waves, group_obj, filt, model_wave
@@ -76,11 +86,12 @@ def model_grad(state, group, patterns, mask, noise, ky, kx, props):
     S = group_obj.shape[1]
     g_obj = numpy.zeros_like(state.object.data)
     n = state.probe.data.shape[-1]
+    tprops = group_props(state, group, ky, kx, props)
     for s in range(S - 1, -1, -1):
         g_obj[s] = osamp.add_view_at_pos(g_obj[s], group_scan, numpy.sum(g * numpy.conj(waves[s]), axis=1))
         g_psi = g * numpy.conj(group_obj[:, s, None])
         if s > 0:
-            g = ifft2(fft2(g_psi) * numpy.conj(props[s - 1])[None, None])
+            g = ifft2(fft2(g_psi) * numpy.conj(tprops[s - 1])[:, None])
         else:
             g = g_psi
     Gk = fft2(g)                                         # spectrum of dL/dpsi0

@@ -32,7 +32,7 @@ class GroupDesc(ctypes.Structure):
         ("gpm", c_vp), ("d_obj", c_vp), ("d_probe", c_vp), ("probe_k", c_vp), ("gom", c_vp),
         ("red", c_vp), ("errors", c_vp), ("aux", c_vp), ("finite_flag", c_vp),
         ("psi_slots", c_i32), ("prop_zero_waves", c_i32), ("grad_obj", c_vp), ("grad_probe", c_vp),
-        ("chi_slices", c_vp),
+        ("chi_slices", c_vp), ("tilt", c_vp), ("slice_dz", c_vp), ("tprops", c_vp),
     ]
 
 
@@ -59,6 +59,7 @@ SIGNATURES = {
                                       c_vp, c_i64, c_vp]),
     "phx_red_doubles": (c_i64, [c_i32, c_i32, c_i32]),
     "phx_group_begin": (c_int, [_P, c_vp]),
+    "phx_tilt_propagators": (c_int, [_P, c_vp]),
     "phx_probe_spectra": (c_int, [_P, c_vp]),
     "phx_group_obj_mag": (c_int, [_P, c_vp]),
     "phx_shift_probes": (c_int, [_P, c_int, c_int, c_vp]),

@@ -81,6 +81,15 @@ __device__ __forceinline__ float2 shift_filter(float sy, float sx, float ky, flo
   return make_float2(cs, sn);
 }
 
+// Propagator j of plane slot b: the shared props[j], or with per-position tilts the slot's
+// tilted copy in tprops (phx_tilt_propagators fills it for the group).  d.tilt is a
+// kernel argument, so the choice is uniform and costs no vector registers.
+__device__ __forceinline__ const float2* prop_of(const phx_group_desc& d, int j, int b) {
+  const size_t NN = (size_t)d.n * d.n;
+  return d.tilt != nullptr ? (const float2*)d.tprops + ((size_t)j * d.bmax + b) * NN
+                           : (const float2*)d.props + (size_t)j * NN;
+}
+
 #define PHX_PLANE_SETUP(N, T)                                                    \
   static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");               \
   __shared__ float2 lds_[Plane<N>::LDS_FLOAT2 + N];                              \
@@ -90,6 +99,26 @@ __device__ __forceinline__ float2 shift_filter(float sy, float sx, float ky, flo
   __syncthreads();
 
 __device__ __forceinline__ bool slot_active(const phx_group_desc& d, int b) { return b < *d.group_size; }
+
+// Tilted propagators of the group's slots (simulation.py:193-222): tprops[j][b][e] =
+// props[j][e] exp(2 pi i dz_j (ty_p ky + tx_p kx)) at scrambled point e = py n + px, with the
+// tangents of slot b's position p (FP64 on the host, float32 here).  The phase is taken
+// in half-turns (sincospif: exact range reduction).  Grid (n^2 / 256, S - 1, bmax).
+__global__ void __launch_bounds__(256) tilt_props_k(phx_group_desc d) {
+  const int b = blockIdx.z, j = blockIdx.y;
+  if (!slot_active(d, b)) return;
+  const int NN = d.n * d.n;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= NN) return;
+  const int p = d.group_pos[b];
+  const float ty = d.tilt[2 * p], tx = d.tilt[2 * p + 1];
+  const float dz2 = 2.0f * d.slice_dz[j];
+  const float h = dz2 * fmaf(ty, d.ky_s[e / d.n], tx * d.kx_s[e % d.n]);
+  float sn, cs;
+  sincospif(h, &sn, &cs);
+  const float2 v = ((const float2*)d.props)[(size_t)j * NN + e];
+  ((float2*)d.tprops)[((size_t)j * d.bmax + b) * NN + e] = cmul(v, make_float2(cs, sn));
+}
 
 // Plane (b, m) of this workgroup in a sweep launch (grid n_modes x bmax).  Workgroups go
 // to the 8 XCDs round-robin by linear id L, so with bmax a multiple of 8 the planes are
@@ -219,7 +248,7 @@ __global__ void __launch_bounds__(T) shift_probes_k(phx_group_desc d, int propag
     atomic_add_f64(&d.red[L.norm + b], tot);
   }
   if (!propagate) return;
-  const float2* prop = (const float2*)d.props;
+  const float2* prop = prop_of(d, 0, b);
   float2* psi1 = (float2*)d.psi + psi_off(d, 1, b, m);
   pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmul(v, prop[py * N + px]); }, d.prop_zero_waves);
   pl.store_rows([&](int yy, int xx, float2 v) { psi1[yy * N + xx] = cscale(v, inv_nn); }, x);
@@ -238,7 +267,7 @@ __global__ void __launch_bounds__(T) forward_slice_k(phx_group_desc d, int i) {
   float2* psi_o = (float2*)d.psi + psi_off(d, i + 1, b, m);
   const float2* obj_i = (const float2*)d.object + i * HW;
   float* gpm_i = d.gpm + i * HW;
-  const float2* prop = (const float2*)d.props + (size_t)i * NN;
+  const float2* prop = prop_of(d, i, b);
   pl.convolve(
       [&](int y, int x) {
         const float2 v = psi_i[y * N + x];
@@ -430,7 +459,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
 
   if (KIND == BK_PROPAGATE) {
     float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
-    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    const float2* prop = prop_of(d, i - 1, b);
     pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
                 [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); }, pin);
   } else if (PROBE) {
@@ -519,7 +548,7 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
     if (inside) step(std::true_type{});
     else step(std::false_type{});
     if (i == S - 1) break;
-    pl.template convolve_midfetch<false>(x, (const float2*)d.props + (size_t)i * NN, d.prop_zero_waves);
+    pl.template convolve_midfetch<false>(x, prop_of(d, i, b), d.prop_zero_waves);
   }
   pl.forward_regs(x);
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
@@ -661,7 +690,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   for (int i = S - 1; i > 0; --i) {
     pl.refresh();
     slice_step(i, std::false_type{}, std::false_type{});
-    pl.template convolve_prefetched<true>(x, (const float2*)d.props + (size_t)(i - 1) * NN, d.prop_zero_waves);
+    pl.template convolve_prefetched<true>(x, prop_of(d, i - 1, b), d.prop_zero_waves);
   }
   pl.refresh();
   slice_step(0, std::integral_constant<bool, PROBE>{}, std::false_type{});
@@ -779,7 +808,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     if (inside) step(std::true_type{});
     else step(std::false_type{});
     if (i == 0) break;
-    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    const float2* prop = prop_of(d, i - 1, b);
     pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); }, d.prop_zero_waves);
   }
   if (!WANT_P0) return;
@@ -1270,7 +1299,7 @@ __global__ void __launch_bounds__(T) epie_backward_k(phx_group_desc d, int i, in
     return cscale(cmulc(cv, o), 1.0f / maxO);
   };
   if (i > 0) {
-    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    const float2* prop = prop_of(d, i - 1, b);
     pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
                 [&](int y, int x, float2 v) { chi_o[y * N + x] = cscale(v, inv_nn); });
   } else if (upd_probe) {
@@ -1413,7 +1442,7 @@ __global__ void __launch_bounds__(T) gradient_backward_k(phx_group_desc d, int i
     return cmulc(gv, o);
   };
   if (i > 0) {
-    const float2* prop = (const float2*)d.props + (size_t)(i - 1) * NN;
+    const float2* prop = prop_of(d, i - 1, b);
     pl.convolve(src, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); },
                 [&](int y, int x, float2 v) { g_o[y * N + x] = cscale(v, inv_nn); });
   } else if (want_probe || want_pos) {
@@ -1558,6 +1587,7 @@ int check_desc(const phx_group_desc* d) {
       !d->params || !d->psi || !d->red)
     return PHX_EINVAL;
   if (d->n_slices > 1 && !d->props) return PHX_EINVAL;
+  if (d->tilt && (!d->slice_dz || !d->tprops)) return PHX_EINVAL;
   return PHX_OK;
 }
 
@@ -1642,6 +1672,17 @@ int phx_group_begin(const phx_group_desc* d, hipStream_t stream) {
   int64_t blocks = (z.na + z.nb + z.nc + z.ne + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(zero4_k, dim3((unsigned)blocks), dim3(256), 0, stream, z);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_tilt_propagators(const phx_group_desc* d, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->tilt || !d->slice_dz || !d->tprops || !d->props || d->n_slices < 2) return PHX_EINVAL;
+  const int NN = d->n * d->n;
+  hipLaunchKernelGGL(tilt_props_k, dim3((unsigned)((NN + 255) / 256), (unsigned)(d->n_slices - 1), (unsigned)d->bmax),
+                     dim3(256), 0, stream, *d);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

@@ -91,6 +91,23 @@ def zero_column_waves(props, n):
     return mask
 
 
+def tilt_tables(tilt, thicknesses, n_pos):
+    """Host tables of the tilted propagators (tilt_propagators, simulation.py:193-222):
+    (tan(1e-3 tilt) as (P, 2) float32 taken in FP64, slice thicknesses dz_j as (S-1)
+    float32), or (None, None) when there is nothing to tilt (no tilts, all-zero tilts,
+    whose ramps are exactly 1, or a single slice)."""
+    if tilt is not None and hasattr(tilt, 'detach'):
+        tilt = tilt.detach().cpu().numpy()
+    dz = numpy.asarray(thicknesses, dtype=numpy.float64)[:-1]
+    if tilt is None or len(dz) == 0 or not numpy.any(numpy.asarray(tilt) != 0):
+        return None, None
+    tilt = numpy.asarray(tilt, dtype=numpy.float64).reshape(-1, 2)
+    if tilt.shape[0] != n_pos:
+        raise ValueError(f"tilt has {tilt.shape[0]} positions, the scan {n_pos}")
+    tan = numpy.ascontiguousarray(numpy.tan(tilt * 1e-3).astype(numpy.float32))
+    return tan, numpy.ascontiguousarray(dz.astype(numpy.float32))
+
+
 class GroupWorkspace:
     """Buffers and descriptors for one engine run over a fixed set of positions."""
 
@@ -144,6 +161,9 @@ class GroupWorkspace:
         self.group_size = torch.zeros(self.G, dtype=i32, device=dev)
         self.start = torch.zeros((self.P, 2), dtype=i32, device=dev)
         self.shift = torch.zeros((self.P, 2), dtype=f32, device=dev)
+        # per-position tilt tangents and slice thicknesses (tilted propagators), or None
+        self.tilt_tan = self.slice_dz = self.tprops = None
+        self.set_tilt(getattr(state, 'tilt', None), state.object.thicknesses)
 
         # work buffers
         S, B, M, NN, H, W = self.S, self.B, self.M, self.NN, self.H, self.W
@@ -220,7 +240,31 @@ class GroupWorkspace:
         d.grad_obj = None if self.grad_obj is None else self.grad_obj.data_ptr()
         d.grad_probe = None if self.grad_probe is None else self.grad_probe.data_ptr()
         d.chi_slices = None if self.chi_slices is None else self.chi_slices.data_ptr()
+        d.tilt = None if self.tilt_tan is None else self.tilt_tan.data_ptr()
+        d.slice_dz = None if self.slice_dz is None else self.slice_dz.data_ptr()
+        d.tprops = None if self.tprops is None else self.tprops.data_ptr()
         return d
+
+    def set_tilt(self, tilt, thicknesses):
+        """Per-position tilts (mrad, (..., 2) in scan order, or None) for the tilted
+        propagators of tilt_propagators (simulation.py:193-222): their tangents
+        tan(1e-3 tilt) are taken in FP64 here and uploaded as float32 with the slice
+        thicknesses.  All-zero tilts give every ramp exactly 1, so they run untilted.
+        Call before the descriptors are built, or rebuild them after."""
+        tan, dz = tilt_tables(tilt, thicknesses, self.P)
+        if tan is None:
+            self.tilt_tan = self.slice_dz = self.tprops = None
+            return
+        self.tilt_tan = self.torch.from_numpy(tan).to(self.device)
+        self.slice_dz = self.torch.from_numpy(dz).to(self.device)
+        self.tprops = self.torch.empty((self.S - 1, self.B, self.NN), dtype=self.torch.complex64, device=self.device)
+
+    def fill_tilt(self, g, stream=None):
+        """The tilted propagators of group slot g's positions into tprops (nothing without
+        tilts).  Every forward model of a group starts with it; the backward kernels that
+        follow read the same group's copies."""
+        if self.tilt_tan is not None:
+            self.call("phx_tilt_propagators", self.desc_refs[g], stream_handle() if stream is None else stream)
 
     # ------------------------------------------------------------------ host tables
     def set_positions(self, obj_sampling, scan):
@@ -275,6 +319,7 @@ class GroupWorkspace:
         c = self.call
         self._begun = None
         c("phx_group_begin", d, st)
+        self.fill_tilt(g, st)
         c("phx_probe_spectra", d, st)
         c("phx_shift_probes", d, 0, int(self.S > 1), st)
         for i in range(1, self.S - 1):
@@ -288,6 +333,7 @@ class GroupWorkspace:
         c = self.call
         self._begun = None
         c("phx_group_begin", d, st)
+        self.fill_tilt(g, st)
         c("phx_probe_spectra", d, st)
         if self.fused and not illum and self.psi_slots >= self.S:
             c("phx_group_forward", d, st)   # all slices in one launch
@@ -348,6 +394,7 @@ class GroupWorkspace:
                 # writes fft2 of the probe it updates, and without a probe update the
                 # probe is the one group 0 transformed
                 c("phx_probe_spectra", d, st)
+            self.fill_tilt(g, st)
             c("phx_group_forward", d, st)
             if update_probe:
                 c("phx_group_obj_mag", d, sh)

@@ -193,14 +193,6 @@ class EPIESolver(_SolverBase):
 SOLVER_CLASSES = {'lsqml': LSQMLSolver, 'epie': EPIESolver}
 
 
-def check_no_tilt(state):
-    """Per-position tilts change every propagator (tilt_propagators, simulation.py:160-222);
-    the MI355X kernels take one set of propagators, so a state with non-zero tilts is
-    refused (all-zero tilts are the untilted problem)."""
-    if state.tilt is not None and numpy.any(numpy.asarray(state.tilt) != 0):
-        raise NotImplementedError("per-position tilts are not supported by the MI355X engines")
-
-
 def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     """Conventional engine (phaser/engines/conventional/run.py:15-122).
 
@@ -220,7 +212,6 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     state = args['state']
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
-    check_no_tilt(state)
 
     noise = props.noise_model
     if noise.name == 'poisson':

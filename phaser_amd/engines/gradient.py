@@ -25,7 +25,7 @@ from ..device import FUSED_MAX_N, GroupWorkspace, host_patterns, stream_handle, 
 from ..groups import GroupManager
 from ..optics import make_propagators
 from ..plan import process_flag, process_schedule, shuffle_groups_flag
-from .conventional import Observer, check_no_tilt
+from .conventional import Observer
 
 logger = logging.getLogger(__name__)
 
@@ -76,6 +76,7 @@ class MultisliceModel:
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         _lib.call("phx_group_begin", d, st)
         self._begun = None
+        ws.fill_tilt(g, st)
         _lib.call("phx_probe_spectra", d, st)
         if self.fused:
             # the whole forward model in one launch; its stored intensity gives the sums
@@ -99,6 +100,9 @@ class MultisliceModel:
         self._begun = None
         if probe_spectra or not self.fused:
             _lib.call("phx_probe_spectra", d, st)
+        # backward() must follow forward() of the same group, so the adjoint finds this
+        # group's tilted propagators in place
+        ws.fill_tilt(g, st)
         self._farfield_pending = False
         if self.fused:
             # every slice in one launch, then the far-field gradient and loss
@@ -353,10 +357,10 @@ def run_engine(args, props, *, recompute=False):
     seed = args.get('seed')
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
-    check_no_tilt(state)
     noise = props.noise_model
     all_vars, group_solvers, iter_solvers = process_solvers(props)
     if 'tilt' in all_vars:
+        # fixed per-position tilts run (tilted propagators); their gradient does not
         raise NotImplementedError("tilt is not a variable of the MI355X gradient engine")
 
     flags = {'probe': process_flag(props.update_probe), 'object': process_flag(props.update_object),

@@ -172,6 +172,7 @@ def simulate_patterns_gpu(truth, osamp, scan, probe, wavelength, psamp, thick, b
     for g in range(G):
         d, st = ws.desc_refs[g], stream_handle()
         _lib.call("phx_group_begin", d, st)
+        ws.fill_tilt(g, st)
         _lib.call("phx_probe_spectra", d, st)
         _lib.call("phx_shift_probes", d, 0, int(ws.S > 1), st)
         for i in range(1, ws.S - 1):

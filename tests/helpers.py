@@ -22,8 +22,10 @@ def to_oracle_state(state, dtype=numpy.float32):
     obj = ost.ObjectState(oobj.ObjectSampling(tuple(os_.shape), os_.sampling, os_.corner, os_.region_min, os_.region_max),
                           numpy.array(numpy.asarray(state.object.data), dtype=cdt),
                           numpy.asarray(state.object.thicknesses, dtype=dtype))
+    tilt = getattr(state, 'tilt', None)
     return ost.ReconsState(iter=ost.IterState(), wavelength=state.wavelength, probe=probe, object=obj,
-                           scan=numpy.array(state.scan, dtype=dtype), tilt=None)
+                           scan=numpy.array(state.scan, dtype=dtype),
+                           tilt=None if tilt is None else numpy.array(tilt, dtype=dtype))
 
 
 def oracle_simulate(truth, osamp, scan, probe, wavelength, psamp, thick, bwlim_frac=2 / 3):

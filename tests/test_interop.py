@@ -219,14 +219,22 @@ def test_position_solvers_take_the_reference_protocol():
     assert sd.init_state(sim) is None and sd.max_step_size == 0.1   # position_correction.py:16
 
 
-def test_engines_refuse_per_position_tilts():
-    """A state with non-zero tilts needs tilted propagators, which the kernels do not
-    take: both engines refuse it before touching the GPU (all-zero tilts are accepted)."""
-    from phaser_amd.engines.conventional import check_no_tilt
-    st = NS(tilt=None)
-    check_no_tilt(st)
-    st.tilt = numpy.zeros((3, 3, 2), dtype=numpy.float32)
-    check_no_tilt(st)
-    st.tilt[1, 1, 0] = 1e-3
-    with pytest.raises(NotImplementedError, match="tilts"):
-        check_no_tilt(st)
+def test_tilt_tables():
+    """Tilted propagators' host tables: nothing to tilt for no tilts, all-zero tilts or a
+    single slice; otherwise tan(1e-3 tilt) per position in scan order (y, x) and the
+    thicknesses of the S-1 propagators; a tilt map of the wrong size is an error."""
+    from phaser_amd.device import tilt_tables
+    th = numpy.array([20., 30., 20.])
+    assert tilt_tables(None, th, 9) == (None, None)
+    z = numpy.zeros((3, 3, 2), dtype=numpy.float32)
+    assert tilt_tables(z, th, 9) == (None, None)
+    t = z.copy()
+    t[1, 2] = (5.0, -2.0)
+    assert tilt_tables(t, th[:1], 9) == (None, None)
+    tan, dz = tilt_tables(t, th, 9)
+    assert tan.dtype == numpy.float32 and tan.shape == (9, 2) and dz.dtype == numpy.float32
+    assert tan[5, 0] == numpy.float32(numpy.tan(5e-3)) and tan[5, 1] == numpy.float32(numpy.tan(-2e-3))
+    assert not numpy.any(numpy.delete(tan, 5, axis=0))
+    assert list(dz) == [20., 30.]
+    with pytest.raises(ValueError, match="positions"):
+        tilt_tables(t, th, 8)
