@@ -104,6 +104,59 @@ def fft_microbench(torch, n=128, batch=2048, reps=5):
             "frac_fp32_peak": tflops / FP32_VECTOR_PEAK_TFLOPS, "gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
 
 
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def mlp_rate(torch, sizes=(4096, 2048, 2048, 2048, 2048), batch=1024, reps=10):
+    """MFMA rate of the learned initializer's network at its training shape (BF16 weights,
+    FP32 accumulate; 4096 -> 2048 x 4, batch 1024): average duration of each GEMM of a
+    forward pass and of a whole training step (forward, loss gradient, backward, Adam),
+    from HIP events on the launch stream, as TFLOP/s and the fraction of the 2.5 PFLOP/s
+    dense BF16 peak.  2 M N K flops per GEMM; a step is 3 GEMM passes less the first
+    layer's input gradient."""
+    from phaser_amd import learned
+    mlp = learned.MLP(sizes, seed=0)
+    g = torch.Generator(device='cuda').manual_seed(0)
+    X = torch.randn((batch, sizes[0]), generator=g, device='cuda').to(torch.bfloat16)
+    T = torch.randn((batch, sizes[-1]), generator=g, device='cuda')
+    mlp.forward(X)
+    mlp.backward(T)
+    mlp.step(1e-6)
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    out = {"sizes": list(sizes), "batch": batch, "peak_tflops": BF16_MFMA_PEAK_TFLOPS, "forward_gemms": []}
+    acts = [X] + [torch.empty((batch, s), dtype=torch.bfloat16, device='cuda') for s in sizes[1:-1]]
+    res = torch.empty((batch, sizes[-1]), dtype=torch.float32, device='cuda')
+    for l in range(len(sizes) - 1):
+        last = l == len(sizes) - 2
+        args = dict(C=res if last else None, Cb=None if last else acts[l + 1], relu=not last)
+        mlp.layer_forward(l, acts[l], **args)
+        e0, e1 = ev(), ev()
+        e0.record()
+        for _ in range(reps):
+            mlp.layer_forward(l, acts[l], **args)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        tf = 2.0 * batch * sizes[l] * sizes[l + 1] / (us * 1e-6) / 1e12
+        out["forward_gemms"].append({"m": batch, "n": sizes[l + 1], "k": sizes[l], "avg_launch_us": us,
+                                     "tflops": tf, "frac": tf / BF16_MFMA_PEAK_TFLOPS})
+    e0, e1 = ev(), ev()
+    e0.record()
+    for _ in range(reps):
+        mlp.forward(X)
+        mlp.backward(T)
+        mlp.step(1e-6)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    fwd = sum(2.0 * batch * a * b for a, b in zip(sizes[:-1], sizes[1:]))
+    step_flops = 3 * fwd - 2.0 * batch * sizes[0] * sizes[1]
+    tf = step_flops / (us * 1e-6) / 1e12
+    out["train_step"] = {"us": us, "gflop": step_flops / 1e9, "tflops": tf, "frac": tf / BF16_MFMA_PEAK_TFLOPS,
+                         "patterns_per_s": batch / (us * 1e-6)}
+    return out
+
+
 def time_kernels(ws, groups, torch, n_groups, constraints=()):
     """Average device duration of each kernel family over one non-graph iteration, from
     HIP events recorded on the stream the kernels are launched on (torch's current
@@ -268,12 +321,18 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a device buffer of this many MiB once (a copy kernel of "
                          "known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--mlp-rate", action="store_true",
+                    help="only measure the learned initializer's GEMM rates at its training shape and exit")
     ap.add_argument("--engines-timing", type=int, default=0, metavar="N",
                     help="only time the gradient engine and plain LSQ-ML on the time-to-NRMSE problem "
                          "(1 and N iterations each) and exit")
     args = ap.parse_args()
     if args.engines_timing:
         engines_timing(args.engines_timing)
+        return
+    if args.mlp_rate:
+        import torch
+        print(json.dumps(mlp_rate(torch)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
@@ -381,6 +440,9 @@ def main():
            "batched_fft2_by_n": [fft_microbench(torch, n=k, batch=(256 << 20) // (8 * k * k))
                                  for k in (64, 128, 192, 256)]}
     log(f"fft rate: {json.dumps(fft)}")
+    # the learned initializer's MFMA rate at its training shape (BF16, 2.5 PFLOP/s peak)
+    mlp = mlp_rate(torch)
+    log(f"mlp rate: {json.dumps(mlp)}")
 
     cpu = None
     if not args.no_cpu:
@@ -426,6 +488,7 @@ def main():
                      "alg_bytes_per_launch": alg, "avg_launch_us": kt[dom]['avg_us'],
                      "whole_iteration_alg_GBs": whole, "whole_iteration_frac": whole / HBM_PEAK_GBS},
         "fft_rate": fft,
+        "mlp_rate": mlp,
         "cpu_baseline": cpu,
         "detector_error_first_last": [errs[0], errs[-1]],
         "time_to_nrmse": ttn,

@@ -408,6 +408,16 @@ int phx_position_finish(const phx_group_desc* d, void* pos_update, hipStream_t s
 int phx_gemm_bf16(const void* A, int64_t lda, int trans_a, const void* B, int64_t ldb, int trans_b, int64_t M,
                   int64_t N, int64_t K, float* C, void* Cb, int64_t ldc, const float* bias, int relu, const void* mask,
                   int64_t ldm, int accumulate, hipStream_t stream);
+/* The NT form of phx_gemm_bf16 (A M x K, B N x K, both k-contiguous) with an optional
+ * transposed BF16 copy of the result, Ct[n * ldct + m]: the double-buffered kernel every
+ * GEMM of the network's training step and inference runs.  M and N multiples of 128, K of
+ * 64; phx_gemm_bf16 routes its NT form with K a multiple of 64 here too. */
+int phx_gemm_bf16_nt(const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K, float* C,
+                     void* Cb, int64_t ldc, void* Ct, int64_t ldct, const float* bias, int relu, const void* mask,
+                     int64_t ldm, int accumulate, hipStream_t stream);
+/* out[c * ldo + r] = in[r * ldi + c] for a rows x cols BF16 array (in != out). */
+int phx_transpose_bf16(const void* in, int64_t ldi, int64_t rows, int64_t cols, void* out, int64_t ldo,
+                       hipStream_t stream);
 /* out[c] = sum_r in[r * ld + c], rows summed in order (FP32). */
 int phx_colsum_f32(const float* in, int64_t rows, int64_t cols, int64_t ld, float* out, hipStream_t stream);
 /* Adam on FP32 master weights w (m, v moments; c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)); the BF16
@@ -419,6 +429,10 @@ int phx_f32_to_bf16(const float* in, void* out, int64_t n, hipStream_t stream);
  * target)^2 in FP64 when loss != NULL. */
 int phx_mse_grad(const float* pred, const float* target, int64_t n, float scale, float* d, void* db, double* loss,
                  hipStream_t stream);
+/* phx_mse_grad over a rows x cols array, also writing the BF16 transpose dbt (cols x rows;
+ * may be NULL, as may d; dbt needs db). */
+int phx_mse_grad_t(const float* pred, const float* target, int64_t rows, int64_t cols, float scale, float* d, void* db,
+                   void* dbt, double* loss, hipStream_t stream);
 /* Network input: out[r] = log(1 + binned(pattern[index[r]]) * inv_scale), the sums of
  * bin x bin pixel blocks, (n/bin)^2 BF16 values per row (n a multiple of bin); patterns
  * (P, n, n) FP32 corner-origin; index NULL means r. */

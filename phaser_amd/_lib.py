@@ -78,6 +78,10 @@ SIGNATURES = {
     "phx_gradient_object_gather_begin": (c_int, [_P, c_int, c_int, _P, c_vp]),
     "phx_gemm_bf16": (c_int, [c_vp, c_i64, c_int, c_vp, c_i64, c_int, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp,
                               c_int, c_vp, c_i64, c_int, c_vp]),
+    "phx_gemm_bf16_nt": (c_int, [c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64,
+                                 c_vp, c_int, c_vp, c_i64, c_int, c_vp]),
+    "phx_transpose_bf16": (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp]),
+    "phx_mse_grad_t": (c_int, [c_vp, c_vp, c_i64, c_i64, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "phx_colsum_f32": (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
     "phx_adam_f32_bf16": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_f32,
                                   c_f32, c_vp]),

@@ -89,6 +89,18 @@ __device__ __forceinline__ const float2* prop_of(const phx_group_desc& d, int j,
   return d.tilt != nullptr ? (const float2*)d.tprops + ((size_t)j * d.bmax + b) * NN
                            : (const float2*)d.props + (size_t)j * NN;
 }
+// The same for a whole slice loop: plane slot b's first propagator and the distance
+// between consecutive ones, taken once before the loop.
+struct PropSeq {
+  const float2* base;
+  size_t stride;
+  __device__ __forceinline__ const float2* operator[](int j) const { return base + (size_t)j * stride; }
+};
+__device__ __forceinline__ PropSeq prop_seq(const phx_group_desc& d, int b) {
+  const size_t NN = (size_t)d.n * d.n;
+  return d.tilt != nullptr ? PropSeq{(const float2*)d.tprops + (size_t)b * NN, (size_t)d.bmax * NN}
+                           : PropSeq{(const float2*)d.props, NN};
+}
 
 #define PHX_PLANE_SETUP(N, T)                                                    \
   static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");               \
@@ -520,6 +532,7 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
   const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
   const float2* Pk = (const float2*)d.probe_k + (size_t)m * NN;
   const bool inside = c.interior(N);
+  const PropSeq props = prop_seq(d, b);
   float nrm = 0.f;  // per-lane FP32 partial of sum |psi_0|^2, FP64 block sum
   auto no_pin = [] {};
 
@@ -548,7 +561,7 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
     if (inside) step(std::true_type{});
     else step(std::false_type{});
     if (i == S - 1) break;
-    pl.template convolve_midfetch<false>(x, prop_of(d, i, b), d.prop_zero_waves);
+    pl.template convolve_midfetch<false>(x, props[i], d.prop_zero_waves);
   }
   pl.forward_regs(x);
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
@@ -590,6 +603,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   const Cut c = cut_of(d, p);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const RedLayout L = red_layout(S, bmax, M);
+  const PropSeq props = prop_seq(d, b);
   // alpha sums: each wave leaves its FP64 partials of a slice step in its own LDS slots
   // (slice i at slot i, the probe sums at slot S) and the slots are summed once, after
   // the sweep, so the slice steps take no block-wide barrier for them.  Deeper stacks
@@ -690,7 +704,7 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   for (int i = S - 1; i > 0; --i) {
     pl.refresh();
     slice_step(i, std::false_type{}, std::false_type{});
-    pl.template convolve_prefetched<true>(x, prop_of(d, i - 1, b), d.prop_zero_waves);
+    pl.template convolve_prefetched<true>(x, props[i - 1], d.prop_zero_waves);
   }
   pl.refresh();
   slice_step(0, std::integral_constant<bool, PROBE>{}, std::false_type{});
@@ -751,6 +765,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   const Cut c = cut_of(d, p);
   const bool inside = c.interior(N);
   const size_t HW = (size_t)d.obj_h * d.obj_w;
+  const PropSeq props = prop_seq(d, b);
   float2 x[E];
   if constexpr (FF == 0) {
     // the stored exit-wave gradient, times n^2 (a power of two: exact) to carry it unscaled
@@ -808,7 +823,7 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     if (inside) step(std::true_type{});
     else step(std::false_type{});
     if (i == 0) break;
-    const float2* prop = prop_of(d, i - 1, b);
+    const float2* prop = props[i - 1];
     pl.convolve_regs(x, [&](int py, int px, float2 v) { return cmulc(v, prop[py * N + px]); }, d.prop_zero_waves);
   }
   if (!WANT_P0) return;

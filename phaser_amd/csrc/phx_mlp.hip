@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -117,13 +119,189 @@ __global__ void __launch_bounds__(256) gemm_bf16_k(const __bf16* __restrict__ A,
       }
 }
 
-// out[c] = sum_r in[r, c] (FP32), one thread per column, rows in order
-__global__ void colsum_k(const float* __restrict__ in, int64_t rows, int64_t cols, int64_t ld, float* __restrict__ out) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// NT GEMM, the form every training and inference GEMM of the network takes: C[M, N] =
+// A[M, K] . B[N, K]^T with both operands k-contiguous (the network keeps a transposed
+// BF16 copy of its weights and of the activations and output gradients the weight
+// gradients need).  256 threads (4 waves, 2 x 2) per 128 x BN tile, each wave a 64 x BN/2
+// block of 32 x 32 v_mfma_f32_32x32x16_bf16 tiles.  K advances 64 per stage.
+//
+// Pipeline.  A stage's MFMAs take a few hundred cycles per SIMD, far less than the
+// latency of its operand loads, and a CU holds one workgroup for these grids, so the
+// loads run D stages ahead: D register slots hold stages s+1 .. s+D while stage s is
+// multiplied out of one of two LDS buffers, and stage s+1 goes from its slot to the
+// other buffer after the MFMAs; one barrier per stage separates the buffers' reads and
+// writes.  (The stage loop is unrolled by D so every slot index is static.)  Row stride
+// 72 BF16 (144 bytes, 36 dwords): the 16-byte stores of a stage (8 lanes per 128-byte
+// row) and the 16-byte MFMA operand reads (8 rows per half-wave group, 4 dwords each)
+// both land on 32 distinct banks.  BN = 64 doubles the workgroups of a 1024 x 2048
+// output (256, one per CU) at the cost of a smaller wave block.  The epilogue of
+// gemm_bf16_k, plus an optional transposed BF16 copy Ct[n * ldct + m].
+constexpr int NT_BM = 128, NT_BK = 64, NT_LDK = NT_BK + 8;
+
+template <int BN, int D>
+__global__ void __launch_bounds__(256) gemm_nt_k(const __bf16* __restrict__ A, int64_t lda,
+                                                 const __bf16* __restrict__ B, int64_t ldb, int K,
+                                                 float* __restrict__ C, __bf16* __restrict__ Cb, int64_t ldc,
+                                                 __bf16* __restrict__ Ct, int64_t ldct, const float* __restrict__ bias,
+                                                 int relu, const __bf16* __restrict__ mask, int64_t ldm,
+                                                 int accumulate) {
+  constexpr int WN = BN / 2, NJ = WN / 32;
+  constexpr int A_LD = NT_BM * NT_BK / 8 / 256, B_LD = BN * NT_BK / 8 / 256;  // 16-byte loads per thread
+  __shared__ __align__(16) __bf16 sA[2][NT_BM * NT_LDK];
+  __shared__ __align__(16) __bf16 sB[2][BN * NT_LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * WN;
+  const int64_t m0 = (int64_t)blockIdx.y * NT_BM, n0 = (int64_t)blockIdx.x * BN;
+  f32x16 acc[2][NJ];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+  // thread -> (row, 8-wide k chunk) of a stage: 8 threads per 64-wide row
+  bf16x8 ra[D][A_LD], rb[D][B_LD];
+  auto gload = [&](int slot, int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int t = 0; t < A_LD; ++t) {
+      const int id = tid + 256 * t, r = id >> 3, kc = (id & 7) * 8;
+      ra[slot][t] = *(const bf16x8*)&A[(m0 + r) * lda + k0 + kc];
+    }
+#pragma unroll
+    for (int t = 0; t < B_LD; ++t) {
+      const int id = tid + 256 * t, r = id >> 3, kc = (id & 7) * 8;
+      rb[slot][t] = *(const bf16x8*)&B[(n0 + r) * ldb + k0 + kc];
+    }
+  };
+  auto sstore = [&](int slot, int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int t = 0; t < A_LD; ++t) {
+      const int id = tid + 256 * t, r = id >> 3, kc = (id & 7) * 8;
+      *(bf16x8*)&sA[buf][r * NT_LDK + kc] = ra[slot][t];
+    }
+#pragma unroll
+    for (int t = 0; t < B_LD; ++t) {
+      const int id = tid + 256 * t, r = id >> 3, kc = (id & 7) * 8;
+      *(bf16x8*)&sB[buf][r * NT_LDK + kc] = rb[slot][t];
+    }
+  };
+  auto compute = [&](int buf) __attribute__((always_inline)) {
+    const __bf16* a_s = sA[buf];
+    const __bf16* b_s = sB[buf];
+#pragma unroll
+    for (int kk = 0; kk < NT_BK; kk += 16) {
+      // lane l holds row (l % 32), k = kk + 8 (l / 32) .. + 7
+      const int kl = kk + 8 * (lane >> 5);
+      bf16x8 a[2], b[NJ];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8*)&a_s[(wm + 32 * i + (lane & 31)) * NT_LDK + kl];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) b[j] = *(const bf16x8*)&b_s[(wn + 32 * j + (lane & 31)) * NT_LDK + kl];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  // The loads and stores run unconditionally (stages past the end reload the last one and
+  // are never multiplied): a branch around them would make the compiler wait for every
+  // load in flight, newest included, before each store.
+  const int nk = K / NT_BK;  // a multiple of D (the host picks D)
+#pragma unroll
+  for (int u = 0; u < D; ++u) gload(u, min(u, nk - 1) * NT_BK);
+  sstore(0, 0);
+  __syncthreads();
+  for (int k0 = 0; k0 < nk; k0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int s = k0 + u;  // the stage multiplied now; slot u held it and is free again
+      gload(u, min(s + D, nk - 1) * NT_BK);
+      // buffer s & 1: u & 1 for even D (k0 a multiple of D)
+      compute(D % 2 == 0 ? (u & 1) : (s & 1));
+      sstore((u + 1) % D, D % 2 == 0 ? ((u + 1) & 1) : ((s + 1) & 1));
+      __syncthreads();
+    }
+  }
+  // epilogue: D element r of lane l is (row 8 (r/4) + 4 (l/32) + r%4, column l % 32)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t row = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t col = n0 + wn + 32 * j + (lane & 31);
+        float v = acc[i][j][r];
+        if (accumulate) v += C[row * ldc + col];
+        if (bias) v += bias[col];
+        if (relu) v = fmaxf(v, 0.f);
+        if (mask && !((float)mask[row * ldm + col] > 0.f)) v = 0.f;
+        if (C) C[row * ldc + col] = v;
+        if (Cb) Cb[row * ldc + col] = to_bf16(v);
+        if (Ct) Ct[col * ldct + row] = to_bf16(v);
+      }
+}
+
+// out[c][r] = in[r][c] for a rows x cols BF16 array (leading dimensions ldi, ldo), 64 x 64
+// tiles through LDS (row stride 66 elements: the column reads of a tile spread over the
+// banks).  Full tiles of 16-byte-aligned rows move 8 elements per access; edge tiles go
+// element by element.
+__global__ void __launch_bounds__(256) transpose_bf16_k(const __bf16* __restrict__ in, int64_t ldi, int64_t rows,
+                                                        int64_t cols, __bf16* __restrict__ out, int64_t ldo) {
+  __shared__ __bf16 t[64 * 66];
+  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+  const bool full = r0 + 64 <= rows && c0 + 64 <= cols && ldi % 8 == 0 && ldo % 8 == 0 &&
+                    reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0;
+  if (full) {
+    for (int v = threadIdx.x; v < 64 * 8; v += 256) {  // row y, elements 8 x8 .. + 7
+      const int y = v >> 3, x8 = (v & 7) * 8;
+      const bf16x8 a = *(const bf16x8*)&in[(r0 + y) * ldi + c0 + x8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) t[y * 66 + x8 + e] = a[e];
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < 64 * 8; v += 256) {  // out row c0 + y, columns r0 + x8 .. + 7
+      const int y = v >> 3, x8 = (v & 7) * 8;
+      bf16x8 a;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) a[e] = t[(x8 + e) * 66 + y];
+      *(bf16x8*)&out[(c0 + y) * ldo + r0 + x8] = a;
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int y = e >> 6, x = e & 63;
+    if (r0 + y < rows && c0 + x < cols) t[y * 66 + x] = in[(r0 + y) * ldi + c0 + x];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int y = e >> 6, x = e & 63;  // out row c0 + y, column r0 + x
+    if (c0 + y < cols && r0 + x < rows) out[(c0 + y) * ldo + r0 + x] = t[x * 66 + y];
+  }
+}
+
+// out[c] = sum_r in[r, c] (FP32): a workgroup of 1024 threads owns 64 columns; thread
+// (strip t, column x) sums rows t, t + 16, t + 32, .. in order (the 64 lanes of a wave
+// read 64 consecutive columns of a row), then the 16 strip sums are added in strip order.
+__global__ void __launch_bounds__(1024) colsum_k(const float* __restrict__ in, int64_t rows, int64_t cols, int64_t ld,
+                                                 float* __restrict__ out) {
+  __shared__ float part[16][64];
+  const int x = threadIdx.x & 63, t = threadIdx.x >> 6;
+  const int64_t c = (int64_t)blockIdx.x * 64 + x;
   float s = 0.f;
-  for (int64_t r = 0; r < rows; ++r) s += in[r * ld + c];
-  out[c] = s;
+  if (c < cols) {
+#pragma unroll 8
+    for (int64_t r = t; r < rows; r += 16) s += in[r * ld + c];
+  }
+  part[t][x] = s;
+  __syncthreads();
+  if (t == 0 && c < cols) {
+    float v = part[0][x];
+    for (int k = 1; k < 16; ++k) v += part[k][x];
+    out[c] = v;
+  }
 }
 
 // Adam (Kingma & Ba) on FP32 master weights, BF16 copy refreshed: bias corrections c1 =
@@ -148,19 +326,21 @@ __global__ void f32_to_bf16_k(const float* __restrict__ in, __bf16* __restrict__
   if (i < n) out[i] = to_bf16(in[i]);
 }
 
-// d = scale (pred - target) (FP32 and BF16), loss partial sum of squares in FP64
+// d = scale (pred - target) (FP32 and BF16), loss partial sum of squares in FP64; a
+// grid-stride loop over at most MSE_BLOCKS workgroups, so the loss takes few atomics
+constexpr int MSE_BLOCKS = 512;
 __global__ void mse_grad_k(const float* __restrict__ pred, const float* __restrict__ target, int64_t n, float scale,
                            float* __restrict__ d, __bf16* __restrict__ db, double* __restrict__ loss) {
   __shared__ double red[256 / 64];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  float e = 0.f;
-  if (i < n) {
-    e = pred[i] - target[i];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float e = pred[i] - target[i];
     const float gi = scale * e;
     if (d) d[i] = gi;
     if (db) db[i] = to_bf16(gi);
+    acc += (double)e * e;
   }
-  const double s = phx::block_sum_f64((double)e * e, red);
+  const double s = phx::block_sum_f64(acc, red);
   if (threadIdx.x == 0 && loss) phx::atomic_add_f64(loss, s);
 }
 
@@ -234,9 +414,71 @@ __global__ void stitch_finish_k(const float* __restrict__ acc_phase, const float
 
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// 128 x 128 tiles when they give a workgroup per CU or more, else 128 x 64
+int launch_gemm_nt(const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K, float* C,
+                   void* Cb, int64_t ldc, void* Ct, int64_t ldct, const float* bias, int relu, const void* mask,
+                   int64_t ldm, int accumulate, hipStream_t stream) {
+  if (M / NT_BM > 65535) return PHX_EUNSUPPORTED;
+  const bool wide = (M / NT_BM) * (N / 128) >= 256;
+  const bool deep = (K / NT_BK) % 4 == 0;  // 4 stages in flight when the stage count allows
+  const __bf16* a = (const __bf16*)A;
+  const __bf16* b = (const __bf16*)B;
+  const __bf16* mk = (const __bf16*)mask;
+#define G(BN, D)                                                                                                      \
+  hipLaunchKernelGGL((gemm_nt_k<BN, D>), dim3((unsigned)(N / BN), (unsigned)(M / NT_BM)), dim3(256), 0, stream, a, lda, \
+                     b, ldb, (int)K, C, (__bf16*)Cb, ldc, (__bf16*)Ct, ldct, bias, relu, mk, ldm, accumulate)
+  if (wide) {
+    if (deep) G(128, 4);
+    else G(128, 1);
+  } else {
+    if (deep) G(64, 4);
+    else G(64, 1);
+  }
+#undef G
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int phx_gemm_bf16_nt(const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K, float* C,
+                     void* Cb, int64_t ldc, void* Ct, int64_t ldct, const float* bias, int relu, const void* mask,
+                     int64_t ldm, int accumulate, hipStream_t stream) {
+  if (!A || !B || (!C && !Cb && !Ct) || M <= 0 || N <= 0 || K <= 0) return PHX_EINVAL;
+  if (M % NT_BM || N % 128 || K % NT_BK) return PHX_EUNSUPPORTED;
+  if ((accumulate && !C) || (mask && ldm < N) || ((C || Cb) && ldc < N) || (Ct && ldct < M)) return PHX_EINVAL;
+  if (lda < K || ldb < K || (lda % 8) || (ldb % 8)) return PHX_EINVAL;
+  return launch_gemm_nt(A, lda, B, ldb, M, N, K, C, Cb, ldc, Ct, ldct, bias, relu, mask, ldm, accumulate, stream);
+}
+
+int phx_transpose_bf16(const void* in, int64_t ldi, int64_t rows, int64_t cols, void* out, int64_t ldo,
+                       hipStream_t stream) {
+  if (!in || !out || rows <= 0 || cols <= 0 || ldi < cols || ldo < rows || in == out) return PHX_EINVAL;
+  if ((rows + 63) / 64 > 65535) return PHX_EUNSUPPORTED;
+  hipLaunchKernelGGL(transpose_bf16_k, dim3((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64)), dim3(256), 0,
+                     stream, (const __bf16*)in, ldi, rows, cols, (__bf16*)out, ldo);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_mse_grad_t(const float* pred, const float* target, int64_t rows, int64_t cols, float scale, float* d, void* db,
+                   void* dbt, double* loss, hipStream_t stream) {
+  if (!pred || !target || rows <= 0 || cols <= 0 || (dbt && !db)) return PHX_EINVAL;
+  if (dbt && (rows + 63) / 64 > 65535) return PHX_EUNSUPPORTED;
+  const int64_t n = rows * cols;
+  hipLaunchKernelGGL(mse_grad_k, dim3(std::min<unsigned>(blocks_for(n, 256), MSE_BLOCKS)), dim3(256), 0, stream, pred,
+                     target, n, scale, d, (__bf16*)db, loss);
+  PHX_CHECK_LAUNCH();
+  if (dbt) {
+    // the transpose through LDS tiles (coalesced both ways)
+    hipLaunchKernelGGL(transpose_bf16_k, dim3((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64)), dim3(256),
+                       0, stream, (const __bf16*)db, cols, rows, cols, (__bf16*)dbt, rows);
+    PHX_CHECK_LAUNCH();
+  }
+  return PHX_OK;
+}
 
 int phx_gemm_bf16(const void* A, int64_t lda, int trans_a, const void* B, int64_t ldb, int trans_b, int64_t M,
                   int64_t N, int64_t K, float* C, void* Cb, int64_t ldc, const float* bias, int relu, const void* mask,
@@ -247,6 +489,8 @@ int phx_gemm_bf16(const void* A, int64_t lda, int trans_a, const void* B, int64_
   if ((!trans_a && lda < K) || (trans_a && lda < M) || (!trans_b && ldb < N) || (trans_b && ldb < K)) return PHX_EINVAL;
   if ((lda % 8) || (ldb % 8)) return PHX_EINVAL;  // 16-byte operand loads
   if (M / BM > 65535) return PHX_EUNSUPPORTED;
+  if (!trans_a && trans_b && K % NT_BK == 0)
+    return launch_gemm_nt(A, lda, B, ldb, M, N, K, C, Cb, ldc, nullptr, 0, bias, relu, mask, ldm, accumulate, stream);
   const dim3 grid((unsigned)(N / BN), (unsigned)(M / BM));
   const __bf16* a = (const __bf16*)A;
   const __bf16* b = (const __bf16*)B;
@@ -265,7 +509,7 @@ int phx_gemm_bf16(const void* A, int64_t lda, int trans_a, const void* B, int64_
 
 int phx_colsum_f32(const float* in, int64_t rows, int64_t cols, int64_t ld, float* out, hipStream_t stream) {
   if (!in || !out || rows < 0 || cols <= 0 || ld < cols) return PHX_EINVAL;
-  hipLaunchKernelGGL(colsum_k, dim3(blocks_for(cols, 256)), dim3(256), 0, stream, in, rows, cols, ld, out);
+  hipLaunchKernelGGL(colsum_k, dim3(blocks_for(cols, 64)), dim3(1024), 0, stream, in, rows, cols, ld, out);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }
@@ -289,8 +533,8 @@ int phx_f32_to_bf16(const float* in, void* out, int64_t n, hipStream_t stream) {
 int phx_mse_grad(const float* pred, const float* target, int64_t n, float scale, float* d, void* db, double* loss,
                  hipStream_t stream) {
   if (!pred || !target || n <= 0) return PHX_EINVAL;
-  hipLaunchKernelGGL(mse_grad_k, dim3(blocks_for(n, 256)), dim3(256), 0, stream, pred, target, n, scale, d,
-                     (__bf16*)db, loss);
+  hipLaunchKernelGGL(mse_grad_k, dim3(std::min<unsigned>(blocks_for(n, 256), MSE_BLOCKS)), dim3(256), 0, stream, pred,
+                     target, n, scale, d, (__bf16*)db, loss);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

@@ -46,6 +46,20 @@ def _pad128(n):
     return (n + 127) // 128 * 128
 
 
+def gemm_nt(A, B, M, N, K, *, C=None, Cb=None, Ct=None, bias=None, relu=False, mask=None, accumulate=False):
+    """C/Cb[M, N] = A[M, K] B[N, K]^T (+ epilogue), Ct = its BF16 transpose, through
+    phx_gemm_bf16_nt (both operands k-contiguous)."""
+    ldc = 0 if C is None and Cb is None else (C if C is not None else Cb).shape[-1]
+    _lib.call("phx_gemm_bf16_nt", _p(A), A.shape[-1], _p(B), B.shape[-1], M, N, K, _p(C), _p(Cb), ldc, _p(Ct),
+              0 if Ct is None else Ct.shape[-1], _p(bias), int(relu), _p(mask), 0 if mask is None else mask.shape[-1],
+              int(accumulate), _stream())
+
+
+def transpose_bf16(A, out):
+    """out (cols x rows) = A^T for a rows x cols BF16 tensor."""
+    _lib.call("phx_transpose_bf16", _p(A), A.shape[-1], A.shape[0], A.shape[-1], _p(out), out.shape[-1], _stream())
+
+
 def gemm(A, B, M, N, K, *, trans_a=False, trans_b=False, C=None, Cb=None, bias=None, relu=False, mask=None,
          accumulate=False):
     """C/Cb[M, N] = op(A) op(B) (+ epilogue), through phx_gemm_bf16."""
@@ -70,7 +84,9 @@ class MLP:
         self.L = len(sizes) - 1
         self.device = device
         rng = numpy.random.default_rng(seed)
-        self.W, self.Wb, self.b = [], [], []
+        # FP32 master weights W (in x out), their BF16 copy Wb and its transpose Wbt
+        # (out x in): every GEMM takes the k-contiguous NT form
+        self.W, self.Wb, self.Wbt, self.b = [], [], [], []
         for l, (a, o) in enumerate(zip(self.sizes[:-1], self.sizes[1:])):
             # He init; the linear output layer starts 10x smaller, or the first steps drive
             # the last hidden layer's ReLUs dead (its active fraction falls to ~5%)
@@ -78,7 +94,9 @@ class MLP:
             w = (rng.standard_normal((a, o)) * scale).astype(numpy.float32)
             self.W.append(torch.from_numpy(w).to(device))
             self.Wb.append(torch.empty((a, o), dtype=torch.bfloat16, device=device))
+            self.Wbt.append(torch.empty((o, a), dtype=torch.bfloat16, device=device))
             _lib.call("phx_f32_to_bf16", _p(self.W[-1]), _p(self.Wb[-1]), a * o, _stream())
+            transpose_bf16(self.Wb[-1], self.Wbt[-1])
             self.b.append(torch.zeros(o, dtype=torch.float32, device=device))
         self.dW = [torch.zeros_like(w) for w in self.W]
         self.db = [torch.zeros_like(b) for b in self.b]
@@ -95,40 +113,54 @@ class MLP:
             torch = _torch()
             bf, f32 = torch.bfloat16, torch.float32
             acts = [None] + [torch.empty((batch, s), dtype=bf, device=self.device) for s in self.sizes[1:-1]]
+            # transposed activations (size x batch) and output gradients: the weight
+            # gradients' k-contiguous operands
+            actsT = [torch.empty((s, batch), dtype=bf, device=self.device) for s in self.sizes[:-1]]
             out = torch.empty((batch, self.sizes[-1]), dtype=f32, device=self.device)
             dz = [torch.empty((batch, s), dtype=f32, device=self.device) for s in self.sizes[1:]]
             dzb = [torch.empty((batch, s), dtype=bf, device=self.device) for s in self.sizes[1:]]
-            self._bufs[batch] = (acts, out, dz, dzb)
+            dzbT = [torch.empty((s, batch), dtype=bf, device=self.device) for s in self.sizes[1:]]
+            self._bufs[batch] = (acts, actsT, out, dz, dzb, dzbT)
         return self._bufs[batch]
 
-    def forward(self, X):
+    def layer_forward(self, l, A, C=None, Cb=None, Ct=None, relu=True):
+        """Z = A W_l + b_l (ReLU'd with relu) into C (FP32), Cb (BF16) and/or its BF16
+        transpose Ct."""
+        gemm_nt(A, self.Wbt[l], A.shape[0], self.sizes[l + 1], self.sizes[l], C=C, Cb=Cb, Ct=Ct, bias=self.b[l],
+                relu=relu)
+
+    def forward(self, X, train=True):
         """X: (batch, sizes[0]) BF16, batch a multiple of 128.  Returns the FP32 output
-        buffer (reused by the next call with the same batch)."""
+        buffer (reused by the next call with the same batch).  train=True also keeps the
+        transposed activations backward() needs."""
         batch = X.shape[0]
-        acts, out, _, _ = self._buffers(batch)
+        acts, actsT, out, _, _, _ = self._buffers(batch)
         acts[0] = X
+        if train:
+            transpose_bf16(X, actsT[0])
         for l in range(self.L):
             last = l == self.L - 1
-            gemm(acts[l], self.Wb[l], batch, self.sizes[l + 1], self.sizes[l], C=out if last else None,
-                 Cb=None if last else acts[l + 1], bias=self.b[l], relu=not last)
+            self.layer_forward(l, acts[l], C=out if last else None, Cb=None if last else acts[l + 1],
+                               Ct=actsT[l + 1] if train and not last else None, relu=not last)
         return out
 
     def backward(self, T):
         """Gradients of mean((out - T)^2) after forward(); the loss sum accumulates in
         self.loss_acc (FP64)."""
         batch = T.shape[0]
-        acts, out, dz, dzb = self._buffers(batch)
+        acts, actsT, out, dz, dzb, dzbT = self._buffers(batch)
         n = out.numel()
-        _lib.call("phx_mse_grad", _p(out), _p(T), n, 2.0 / n, _p(dz[-1]), _p(dzb[-1]), _p(self.loss_acc), _stream())
+        _lib.call("phx_mse_grad_t", _p(out), _p(T), batch, self.sizes[-1], 2.0 / n, _p(dz[-1]), _p(dzb[-1]),
+                  _p(dzbT[-1]), _p(self.loss_acc), _stream())
         for l in range(self.L - 1, -1, -1):
-            # dW_l = A_l^T dZ_l (A_l stored batch x in: the transposed operand)
-            gemm(acts[l], dzb[l], self.sizes[l], self.sizes[l + 1], batch, trans_a=True, C=self.dW[l])
+            # dW_l = A_l^T dZ_l: (in x batch) times (out x batch)^T
+            gemm_nt(actsT[l], dzbT[l], self.sizes[l], self.sizes[l + 1], batch, C=self.dW[l])
             _lib.call("phx_colsum_f32", _p(dz[l]), batch, self.sizes[l + 1], self.sizes[l + 1], _p(self.db[l]),
                       _stream())
             if l > 0:
-                # dZ_{l-1} = (dZ_l W_l^T) * relu'(Z_{l-1}); W_l stored in x out = N x K
-                gemm(dzb[l], self.Wb[l], batch, self.sizes[l], self.sizes[l + 1], trans_b=True, C=dz[l - 1],
-                     Cb=dzb[l - 1], mask=acts[l])
+                # dZ_{l-1} = (dZ_l W_l^T) * relu'(Z_{l-1}): W_l (in x out) is the N x K operand
+                gemm_nt(dzb[l], self.Wb[l], batch, self.sizes[l], self.sizes[l + 1], C=dz[l - 1], Cb=dzb[l - 1],
+                        Ct=dzbT[l - 1], mask=acts[l])
 
     def step(self, lr, b1=0.9, b2=0.999, eps=1e-8, weight_decay=0.0):
         self.t += 1
@@ -138,6 +170,7 @@ class MLP:
         for l in range(self.L):
             _lib.call("phx_adam_f32_bf16", _p(self.W[l]), _p(self.Wb[l]), _p(self.dW[l]), _p(self.mW[l]),
                       _p(self.vW[l]), self.W[l].numel(), lr, b1, b2, eps, c1, c2, weight_decay, st)
+            transpose_bf16(self.Wb[l], self.Wbt[l])
             _lib.call("phx_adam_f32_bf16", _p(self.b[l]), None, _p(self.db[l]), _p(self.mb[l]), _p(self.vb[l]),
                       self.b[l].numel(), lr, b1, b2, eps, c1, c2, 0.0, st)
 
@@ -154,7 +187,7 @@ class MLP:
                 xb[:r1 - r0] = X[r0:r1]
             else:
                 xb = X[r0:r1]
-            res[r0:r1] = self.forward(xb)[:r1 - r0]
+            res[r0:r1] = self.forward(xb, train=False)[:r1 - r0]
         return res
 
     def state_dict(self):
@@ -164,6 +197,7 @@ class MLP:
         for l in range(self.L):
             self.W[l].copy_(d['W'][l])
             _lib.call("phx_f32_to_bf16", _p(self.W[l]), _p(self.Wb[l]), self.W[l].numel(), _stream())
+            transpose_bf16(self.Wb[l], self.Wbt[l])
             self.b[l].copy_(d['b'][l])
 
 
@@ -312,6 +346,7 @@ def train(mlp, X, T, *, epochs, batch=1024, lr=1e-4, seed=0, log=None, standardi
         mlp.W[L].mul_(sd)
         mlp.b[L].mul_(sd).add_(mu)
         _lib.call("phx_f32_to_bf16", _p(mlp.W[L]), _p(mlp.Wb[L]), mlp.W[L].numel(), _stream())
+        transpose_bf16(mlp.Wb[L], mlp.Wbt[L])
         return hist
     rows = X.shape[0]
     batch = min(batch, rows // 128 * 128)

@@ -1,5 +1,6 @@
 """Learned-initializer kernels on the GPU against the CPU oracle (oracle/mlp.py):
-BF16 MFMA GEMMs in all four transpose forms with their epilogues, a full training step
+BF16 MFMA GEMMs in all four transpose forms with their epilogues, the double-buffered NT
+kernel with its transposed output, the BF16 transpose, a full training step
 (forward, backward, Adam) of a small MLP, the features, patch targets and stitching, and
 training that lowers the loss."""
 import numpy
@@ -51,6 +52,48 @@ def test_gemm_epilogue_bias_relu_mask_accumulate_bf16_out():
     numpy.testing.assert_array_equal(Cb.float().cpu().numpy(), om.round_bf16(c))
 
 
+@pytest.mark.parametrize('M,N,K,epi', [(128, 128, 64, False), (256, 384, 192, True), (2048, 2048, 256, True)])
+def test_gemm_nt_double_buffered_with_transposed_copy(M, N, K, epi):
+    """phx_gemm_bf16_nt: one K stage and several (the two LDS buffers alternating), the
+    128 x 64 tiles (fewer than 256 workgroups of 128 x 128) and the 128 x 128 ones
+    (2048 x 2048), with and without the epilogue; Cb and the transposed Ct are the BF16
+    rounding of C bit for bit."""
+    import torch
+    from phaser_amd.learned import gemm_nt
+    rng = numpy.random.default_rng(M + N + K)
+    A = om.round_bf16(rng.normal(size=(M, K)))
+    B = om.round_bf16(rng.normal(size=(N, K)))
+    ref = A.astype(numpy.float64) @ B.T.astype(numpy.float64)
+    C0 = rng.normal(size=(M, N)).astype(numpy.float32)
+    kw = {}
+    if epi:
+        bias = rng.normal(size=N).astype(numpy.float32)
+        mask = om.round_bf16(rng.normal(size=(M, N)))
+        ref = numpy.maximum(ref + C0 + bias, 0.0) * (mask > 0)
+        kw = dict(bias=torch.from_numpy(bias).cuda(), relu=True, mask=_bf(torch, mask), accumulate=True)
+    C = torch.from_numpy(C0).cuda()
+    Cb = torch.empty((M, N), dtype=torch.bfloat16, device='cuda')
+    Ct = torch.empty((N, M), dtype=torch.bfloat16, device='cuda')
+    gemm_nt(_bf(torch, A), _bf(torch, B), M, N, K, C=C, Cb=Cb, Ct=Ct, **kw)
+    c = C.cpu().numpy()
+    err = rel_err(c, ref)
+    print(f"gemm_nt {M}x{N}x{K} epilogue={epi}: rel err {err:.2e}", flush=True)
+    assert err < 1e-5
+    numpy.testing.assert_array_equal(Cb.float().cpu().numpy(), om.round_bf16(c))
+    numpy.testing.assert_array_equal(Ct.float().cpu().numpy(), om.round_bf16(c).T)
+
+
+def test_transpose_bf16():
+    import torch
+    from phaser_amd.learned import transpose_bf16
+    rng = numpy.random.default_rng(3)
+    for rows, cols in ((128, 384), (100, 70), (4096, 2048)):
+        A = om.round_bf16(rng.normal(size=(rows, cols)))
+        out = torch.empty((cols, rows), dtype=torch.bfloat16, device='cuda')
+        transpose_bf16(_bf(torch, A), out)
+        numpy.testing.assert_array_equal(out.float().cpu().numpy(), A.T)
+
+
 def test_gemm_rejects_unsupported_shapes():
     import torch
     from phaser_amd.learned import gemm
@@ -93,6 +136,7 @@ def test_mlp_training_step_matches_oracle():
     for l in range(3):
         assert rel_err(mlp.W[l].cpu().numpy(), w_ref[l]) < 1e-6
         numpy.testing.assert_array_equal(mlp.Wb[l].float().cpu().numpy(), om.round_bf16(mlp.W[l].cpu().numpy()))
+        numpy.testing.assert_array_equal(mlp.Wbt[l].float().cpu().numpy(), om.round_bf16(mlp.W[l].cpu().numpy()).T)
 
 
 def test_features_patches_and_stitch():
