@@ -259,6 +259,29 @@ def time_to_nrmse_24_slices(args, data, log):
     return res
 
 
+def learned_at_scale(n_objects, epochs):
+    """The learned initializer trained at scale: `n_objects` synthetic objects (seeded
+    Gaussian-atom lattices, 3-5 slices, 128x128 x 4 modes, 16x16 positions) reconstructed
+    by 40 iterations of plain LSQ-ML on the GPU, the 4096 -> 2048 x 4 network trained on
+    every pattern (batch 1024, Adam), then the time to object NRMSE 0.05 on a held-out
+    problem (learned init + gradient engine, flat start + gradient engine, plain LSQ-ML).
+    Prints one JSON line."""
+    import torch
+    from phaser_amd import learned
+    mlp, info = learned.train_on_reconstructions(n_objects, niter=40, epochs=epochs, lr=1e-4, log=log)
+    steps = epochs * (info['rows'] // 1024)
+    info['train_steps'] = steps
+    info['train_patterns_per_s'] = steps * 1024 / info['train_seconds']
+    log(f"trained: {json.dumps(info)}")
+    ttn = learned.time_to_nrmse(learned.evaluation_config(), mlp, threshold=0.05, max_iter=60, lr_object=30.0,
+                                log=log)
+    for k in ('plain_lsqml', 'learned_gradient', 'flat_gradient'):
+        h = ttn[k].pop('history')
+        ttn[k]['nrmse_by_iteration'] = [e for _t, _i, e in h]
+    print(json.dumps({"learned_initializer": {"training": info, "time_to_nrmse": ttn,
+                                              "mlp_rate": mlp_rate(torch)}}), flush=True)
+
+
 def engines_timing(niter):
     """Wall time of the gradient engine (SGD with momentum on the object, the learned
     comparison's settings) and of plain LSQ-ML on the time-to-NRMSE problem, for 1 and
@@ -311,7 +334,7 @@ def main():
     ap.add_argument("--cpu-groups", type=int, default=8)
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-initializer time-to-NRMSE comparison")
-    ap.add_argument("--learn-objects", type=int, default=48)
+    ap.add_argument("--learn-objects", type=int, default=400)
     ap.add_argument("--ttn", action="store_true",
                     help="with --workload 192x8-s24: after the throughput measurement, train the learned "
                          "initializer on the gradient engine's reconstructions of 32x32-position objects with "
@@ -321,6 +344,12 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a device buffer of this many MiB once (a copy kernel of "
                          "known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--learned-scale", type=int, default=0, metavar="OBJECTS",
+                    help="only train the learned initializer on this many plain-LSQ-ML reconstructions of "
+                         "16x16-position synthetic objects (3-5 slices, 128x128 x 4 modes: 256 patterns each; "
+                         "2000 gives the 500k-pattern training set), evaluate it on the time-to-NRMSE problem and "
+                         "exit")
+    ap.add_argument("--learned-epochs", type=int, default=10)
     ap.add_argument("--mlp-rate", action="store_true",
                     help="only measure the learned initializer's GEMM rates at its training shape and exit")
     ap.add_argument("--engines-timing", type=int, default=0, metavar="N",
@@ -333,6 +362,9 @@ def main():
     if args.mlp_rate:
         import torch
         print(json.dumps(mlp_rate(torch)), flush=True)
+        return
+    if args.learned_scale:
+        learned_at_scale(args.learned_scale, args.learned_epochs)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "

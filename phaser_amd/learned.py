@@ -568,6 +568,8 @@ def train_on_reconstructions(n_objects, *, base=None, niter=10, epochs=20, lr=1e
         X, T = training_examples(cfg, niter=niter, method=method, gradient_opts=gradient_opts)
         Xs.append(X)
         Ts.append(T)
+        if log and (k + 1) % 100 == 0:
+            log(f"training set: {k + 1} of {n_objects} objects reconstructed ({time.perf_counter() - t0:.0f} s)")
     X = torch.cat(Xs)
     T = torch.cat(Ts)
     torch.cuda.synchronize()
