@@ -473,8 +473,8 @@ def main():
                                  for k in (64, 128, 192, 256)]}
     log(f"fft rate: {json.dumps(fft)}")
     # the learned initializer's MFMA rate at its training shape (BF16, 2.5 PFLOP/s peak)
-    mlp = mlp_rate(torch)
-    log(f"mlp rate: {json.dumps(mlp)}")
+    gemm_rate = mlp_rate(torch)
+    log(f"mlp rate: {json.dumps(gemm_rate)}")
 
     cpu = None
     if not args.no_cpu:
@@ -520,7 +520,7 @@ def main():
                      "alg_bytes_per_launch": alg, "avg_launch_us": kt[dom]['avg_us'],
                      "whole_iteration_alg_GBs": whole, "whole_iteration_frac": whole / HBM_PEAK_GBS},
         "fft_rate": fft,
-        "mlp_rate": mlp,
+        "mlp_rate": gemm_rate,
         "cpu_baseline": cpu,
         "detector_error_first_last": [errs[0], errs[-1]],
         "time_to_nrmse": ttn,
