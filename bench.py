@@ -321,14 +321,80 @@ def engines_timing(niter):
             log(f"{name}: {(t[niter] - t[1]) / (niter - 1) * 1e3:.2f} ms per iteration")
 
 
+def gradient_engine_workload(args):
+    """The gradient engine at its configuration's full shape: 16 slices of 1030^2 (the
+    1024^2 field of view of 128 x 128 positions at 7 px plus the probe's margin), a
+    128x128 probe x 6 modes, 16,384 positions with +-1 px jitter,
+    Poisson 5e4; Adam (Nesterov) on object and probe per group, momentum SGD on the
+    positions per iteration (position correction on), groups of 64.  `warmup` untimed
+    iterations, then `steps` timed ones (one engine run; each iteration closes with a
+    device synchronisation in the observer).  Prints one JSON line."""
+    import torch
+    from phaser_amd.engines import gradient as geng
+    from phaser_amd.plan import AdamSolverPlan, GradientEnginePlan, SGDSolverPlan
+    from phaser_amd.synthetic import SyntheticConfig, make_dataset
+    cfg = SyntheticConfig(n=128, n_modes=6, n_slices=16, scan_shape=(128, 128), scan_step_px=7.0, jitter_px=1.0,
+                          dose=5e4)
+    log("generating synthetic data (GPU forward model + Poisson noise)")
+    state0, pats, _truth = make_dataset(cfg)
+    log(f"data ready: patterns {pats.patterns.shape}, object {state0.object.data.shape}")
+    solvers = {'object': AdamSolverPlan(learning_rate=5e-3, nesterov=True),
+               'probe': AdamSolverPlan(learning_rate=0.1, nesterov=True),
+               'positions': SGDSolverPlan(learning_rate=0.5, momentum=0.9)}
+    plan = GradientEnginePlan(noise_model='amplitude', niter=args.warmup + args.steps, grouping=64, solvers=solvers,
+                              update_positions=True)
+    stamps = []
+
+    class Clock:
+        wants_groups = False
+
+        def init_engine(self, *a, **k):
+            pass
+
+        def start_engine(self, *a, **k):
+            torch.cuda.synchronize()
+            stamps.append(time.perf_counter())
+
+        def update_group(self, *a, **k):
+            pass
+
+        def update_iteration(self, state, i, n, error=None):
+            torch.cuda.synchronize()
+            stamps.append(time.perf_counter())
+            log(f"iteration {i}: loss {error}")
+
+        def finish_engine(self, *a, **k):
+            pass
+
+    out = geng.run_engine({'data': pats, 'state': state0.to_device(), 'seed': 0, 'dtype': numpy.float32,
+                           'observer': Clock()}, plan)
+    P = int(numpy.prod(cfg.scan_shape))
+    dt = stamps[-1] - stamps[args.warmup]
+    value = P * args.steps / dt
+    losses = [float(v) for v in out.progress.detector_errors]
+    line = {"metric": "gradient-engine patterns·iter/s (adjoint kernels, position correction on), 1 MI355X",
+            "value": value, "unit": "patterns·iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"gradient engine: {cfg.n_slices} slices of "
+                                   f"{state0.object.data.shape[1]}x{state0.object.data.shape[2]} complex64, "
+                                   f"{cfg.n}x{cfg.n} probe x {cfg.n_modes} modes, {P} patterns (Poisson 5e4), "
+                                   "groups of 64, Adam on object and probe, momentum SGD on positions",
+                       "stored_waves": True},
+            "loss_first_last": [losses[0], losses[-1]]}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-graphs", action="store_true")
-    ap.add_argument("--workload", choices=("128x4-s8", "192x8-s24"), default="128x4-s8",
-                    help="192x8-s24: the same LSQ-ML measurement at 24 slices, 192x192 x 8 modes, 36864 "
+    ap.add_argument("--workload", choices=("128x4-s8", "192x8-s24", "128x6-s16-grad"), default="128x4-s8",
+                    help="128x6-s16-grad: the gradient engine at 16 slices of 1030^2, 128x128 x 6 modes, 16384 "
+                         "positions at 7 px, position correction on (its own JSON line); "
+                         "192x8-s24: the same LSQ-ML measurement at 24 slices, 192x192 x 8 modes, 36864 "
                          "positions (no CPU baseline; its time-to-NRMSE leg with --ttn)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-groups", type=int, default=8)
@@ -370,6 +436,9 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
               "build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "128x6-s16-grad":
+        gradient_engine_workload(args)
+        return
 
     import torch
     if not torch.cuda.is_available():
