@@ -20,6 +20,7 @@
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int BM = 128, BN = 128, BK = 32;
@@ -224,24 +225,61 @@ __global__ void __launch_bounds__(256) gemm_nt_k(const __bf16* __restrict__ A, i
       __syncthreads();
     }
   }
-  // epilogue: D element r of lane l is (row 8 (r/4) + 4 (l/32) + r%4, column l % 32)
+  // epilogue: D element r of lane l is (row 8 (r/4) + 4 (l/32) + r%4, column l % 32).  Each
+  // option is a uniform branch around a whole 16-element loop, so a tile's loads (C to
+  // accumulate, the mask) are issued together rather than one element at a time; a lane's
+  // bias is one value per column block.  Ct gets the lane's 4 consecutive rows of a
+  // column as one 8-byte store.
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < NJ; ++j) {
+    const int64_t col = n0 + wn + 32 * j + (lane & 31);
+    const float bv = bias ? bias[col] : 0.f;
 #pragma unroll
-    for (int j = 0; j < NJ; ++j)
+    for (int i = 0; i < 2; ++i) {
+      const int64_t row0 = m0 + wm + 32 * i + 4 * (lane >> 5);
+      auto row = [&](int r) { return row0 + 8 * (r >> 2) + (r & 3); };
+      float v[16];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int64_t row = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int64_t col = n0 + wn + 32 * j + (lane & 31);
-        float v = acc[i][j][r];
-        if (accumulate) v += C[row * ldc + col];
-        if (bias) v += bias[col];
-        if (relu) v = fmaxf(v, 0.f);
-        if (mask && !((float)mask[row * ldm + col] > 0.f)) v = 0.f;
-        if (C) C[row * ldc + col] = v;
-        if (Cb) Cb[row * ldc + col] = to_bf16(v);
-        if (Ct) Ct[col * ldct + row] = to_bf16(v);
+      for (int r = 0; r < 16; ++r) v[r] = acc[i][j][r];
+      if (accumulate) {
+        float c[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c[r] = C[row(r) * ldc + col];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] += c[r];
       }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] += bv;
+      if (relu) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = fmaxf(v[r], 0.f);
+      }
+      if (mask) {
+        __bf16 mk[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mk[r] = mask[row(r) * ldm + col];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = ((float)mk[r] > 0.f) ? v[r] : 0.f;
+      }
+      if (C) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) C[row(r) * ldc + col] = v[r];
+      }
+      if (Cb) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Cb[row(r) * ldc + col] = to_bf16(v[r]);
+      }
+      if (Ct) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          bf16x4 t;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) t[e] = to_bf16(v[4 * q + e]);
+          *(bf16x4*)&Ct[col * ldct + row(4 * q)] = t;
+        }
+      }
+    }
+  }
 }
 
 // out[c][r] = in[r][c] for a rows x cols BF16 array (leading dimensions ldi, ldo), 64 x 64
@@ -449,6 +487,7 @@ int phx_gemm_bf16_nt(const void* A, int64_t lda, const void* B, int64_t ldb, int
   if (!A || !B || (!C && !Cb && !Ct) || M <= 0 || N <= 0 || K <= 0) return PHX_EINVAL;
   if (M % NT_BM || N % 128 || K % NT_BK) return PHX_EUNSUPPORTED;
   if ((accumulate && !C) || (mask && ldm < N) || ((C || Cb) && ldc < N) || (Ct && ldct < M)) return PHX_EINVAL;
+  if (Ct && (ldct % 4 || reinterpret_cast<uintptr_t>(Ct) % 8)) return PHX_EINVAL;  // 8-byte transposed stores
   if (lda < K || ldb < K || (lda % 8) || (ldb % 8)) return PHX_EINVAL;
   return launch_gemm_nt(A, lda, B, ldb, M, N, K, C, Cb, ldc, Ct, ldct, bias, relu, mask, ldm, accumulate, stream);
 }
