@@ -110,7 +110,8 @@ typedef struct phx_group_desc {
   void* chi_a;               /* (bmax, M, n, n) complex64 */
   void* chi_b;               /* (bmax, M, n, n) complex64 */
   void* farfield;            /* (bmax, M, n*n) complex64, scrambled exit-wave spectra */
-  float* intensity;          /* (bmax, n*n) float32 model intensity, scrambled */
+  float* intensity;          /* (bmax, M, n*n) float32 per-mode model intensity |F_m|^2 / n^2,
+                                scrambled; consumers add the M terms in mode order */
   float* gpm;                /* (S, H, W) float32 group illumination */
   void* d_obj;               /* (H, W) complex64 object update accumulator (kept zero) */
   void* d_probe;             /* (M, n*n) complex64 probe update spectrum (kept zero) */
@@ -152,7 +153,7 @@ int phx_tilt_propagators(const phx_group_desc* d, hipStream_t stream);
 /* Doubles needed by phx_group_desc.red. */
 int64_t phx_red_doubles(int32_t n_slices, int32_t bmax, int32_t n_modes);
 
-/* Zero the per-group accumulators (gom, intensity, red, errors).  gpm, used only when a
+/* Zero the per-group accumulators (gom, red, errors).  gpm, used only when a
  * forward kernel is called with illum != 0, is the caller's to clear. */
 int phx_group_begin(const phx_group_desc* d, hipStream_t stream);
 
@@ -174,8 +175,8 @@ int phx_shift_probes(const phx_group_desc* d, int illum, int propagate, hipStrea
  * illumination of slice i when illum != 0 (solvers.py:210-222). */
 int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t stream);
 
-/* Exit wave (solvers.py:231-240): farfield = FFT(psi[S-1] * O_{S-1}), intensity = sum_m
- * |fft2|^2; illumination of slice S-1 when illum != 0.  Then the amplitude/Anscombe
+/* Exit wave (solvers.py:231-240): farfield = FFT(psi[S-1] * O_{S-1}), intensity[b][m] =
+ * |fft2|^2 / n^2 per mode (summed over m in mode order where read); illumination of slice S-1 when illum != 0.  Then the amplitude/Anscombe
  * modulus constraint (noise_models.py:47-63) chi_a = ifft2(update * Psi) and, when
  * calc_error != 0, errors[b] = sqrt(nansum((I - D)^2)) (solvers.py:237, FP64 sum). */
 int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t stream);
@@ -183,8 +184,8 @@ int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t
 /* Fused forward model of a group (simulation.py:241-285, solvers.py:202-236): one launch
  * carries each (slot, mode) plane through every slice with the wave in registers:
  * psi[0] = ifft2(fft2(probe) * shift_filter(b)), psi[i+1] = ifft2(fft2(psi[i] * O_i) *
- * prop_i) stored for every i, then farfield = FFT(psi[S-1] * O_{S-1}) and intensity +=
- * |farfield|^2 / n^2 (intensity zeroed by phx_group_begin), plus the per-slot norm
+ * prop_i) stored for every i, then farfield = FFT(psi[S-1] * O_{S-1}) and intensity[b][m] =
+ * |farfield|^2 / n^2 (plain stores, one plane per mode), plus the per-slot norm
  * sum_m |psi[0]|^2.  Replaces phx_shift_probes + phx_forward_slice + the first half of
  * phx_farfield.  Needs psi_slots covering every slice. */
 int phx_group_forward(const phx_group_desc* d, hipStream_t stream);
@@ -224,8 +225,8 @@ int phx_lsqml_object_update_slices(const phx_group_desc* d, int first, int count
 /* phx_lsqml_object_update_slices(d, first, count) and phx_group_begin(next) in one launch:
  * the update of group d also zeroes the accumulators of the group `next` starts from (the
  * fresh per-group arrays of lsqml_run, solvers.py:171-200).  Valid when d's update reads
- * none of next's buffers: next->red must differ from d->red; gom, intensity and errors are
- * not read by the update.  Saves the separate clear launch between a group's backward
+ * none of next's buffers: next->red must differ from d->red; gom and errors are not read
+ * by the update.  Saves the separate clear launch between a group's backward
  * sweep and its object update. */
 int phx_lsqml_object_update_slices_begin(const phx_group_desc* d, int first, int count, const phx_group_desc* next,
                                          hipStream_t stream);
@@ -251,12 +252,17 @@ int phx_lsqml_probe_apply_strips_static(const phx_group_desc* d, hipStream_t str
 /* Capacity, in complex64 elements, of phx_lsqml_probe_apply_strips_static's work array. */
 int64_t phx_probe_strip_scratch_elems(void);
 
-/* ePIE backward step at slice i (solvers.py:459-485), maxima first, then the update.  At
- * slice 0 with update_probe each plane's probe-update spectrum fft2(P_upd) conj(filt_b)
- * goes to its farfield slot (the far-field step is done with it). */
+/* ePIE backward step at slice i (solvers.py:459-485), maxima first (into aux), then the
+ * back-propagation of chi through slice i.  At slice 0 with update_probe each plane's
+ * probe-update spectrum fft2(P_upd) conj(filt_b) goes to its farfield slot (the far-field
+ * step is done with it).  update_object is accepted for the interface and needs nothing
+ * here: phx_epie_object_apply gathers the object update afterwards. */
 int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int update_probe,
                       hipStream_t stream);
-/* O_i += d_obj; d_obj = 0. */
+/* ePIE object update of slice i (solvers.py:466-477): O_i += beta_O/S sum_b (sum_m chi_i
+ * conj(psi_i)) / max(sum_m |psi_i|^2)_b over the group's cutouts, gathered per object tile
+ * in group order (no atomics).  Call it after phx_epie_backward(slice) and before
+ * phx_epie_backward(slice - 1), whose output overwrites chi_i. */
 int phx_epie_object_apply(const phx_group_desc* d, int slice, hipStream_t stream);
 /* d_probe = sum_b farfield[b] in position order (phx_probe_spectrum_reduce), then
  * P += beta_P ifft2(d_probe / B); d_probe = 0. */

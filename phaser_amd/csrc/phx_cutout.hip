@@ -334,6 +334,7 @@ int phx_scramble(const void* in, void* out, int64_t batch, int64_t n, int is_com
     case 64: return launch_scramble<64>(in, out, batch, is_complex, unscramble, stream);
     case 128: return launch_scramble<128>(in, out, batch, is_complex, unscramble, stream);
     case 192: return launch_scramble<192>(in, out, batch, is_complex, unscramble, stream);
+    case 256: return launch_scramble<256>(in, out, batch, is_complex, unscramble, stream);
     default: return PHX_EUNSUPPORTED;
   }
 }

@@ -102,6 +102,64 @@ __device__ __forceinline__ PropSeq prop_seq(const phx_group_desc& d, int b) {
                            : PropSeq{(const float2*)d.props, NN};
 }
 
+// Model intensity sum_m |F_m|^2 / n^2 at spectrum point e of slot b.  The forward kernels
+// leave each mode's term in its own plane (intensity[b][m]) with plain stores, and the
+// terms are added here in mode order, as numpy's sum over the mode axis adds them: the
+// result does not depend on the order in which the modes' workgroups ran.
+__device__ __forceinline__ float model_intensity(const phx_group_desc& d, int b, int e) {
+  const size_t NN = (size_t)d.n * d.n;
+  const float* I = d.intensity + (size_t)b * d.n_modes * NN + e;
+  float s = I[0];
+  for (int k = 1; k < d.n_modes; ++k) s += I[k * NN];
+  return s;
+}
+
+// model_intensity at every column-phase point of this lane (register k: py = QL k + q,
+// px = rc).  Up to four modes' loads for all the points are issued before the first add
+// (one memory round trip, not one per mode); the adds keep mode order.
+template <int N>
+__device__ __forceinline__ void model_intensity_cols(const phx_group_desc& d, int b, int q, int rc,
+                                                     float (&Iv)[Plane<N>::E]) {
+  constexpr int NN = N * N, E = Plane<N>::E, QL = Plane<N>::QL;
+  const int M = d.n_modes;
+  const float* I = d.intensity + (size_t)b * M * NN + q * N + rc;
+  if (M == 1) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) Iv[k] = I[k * QL * N];
+    return;
+  }
+  if (M <= 4) {
+    // modes 1..3 (a mode past M re-reads mode 0 and is not added)
+    const float* I1 = I + (M > 1 ? NN : 0);
+    const float* I2 = I + (M > 2 ? 2 * NN : 0);
+    const float* I3 = I + (M > 3 ? 3 * NN : 0);
+    float t1[E], t2[E], t3[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      Iv[k] = I[k * QL * N];
+      t1[k] = I1[k * QL * N];
+      t2[k] = I2[k * QL * N];
+      t3[k] = I3[k * QL * N];
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      float s = Iv[k];
+      if (M > 1) s += t1[k];
+      if (M > 2) s += t2[k];
+      if (M > 3) s += t3[k];
+      Iv[k] = s;
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < E; ++k) Iv[k] = I[k * QL * N];
+  for (int mm = 1; mm < M; ++mm) {
+    const float* Im = I + (size_t)mm * NN;
+#pragma unroll
+    for (int k = 0; k < E; ++k) Iv[k] += Im[k * QL * N];
+  }
+}
+
 #define PHX_PLANE_SETUP(N, T)                                                    \
   static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");               \
   __shared__ float2 lds_[Plane<N>::LDS_FLOAT2 + N];                              \
@@ -306,8 +364,7 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d) {
   const float2* obj_s = (const float2*)d.object + s * HW;
   float* gpm_s = d.gpm + s * HW;
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
-  float* I = d.intensity + (size_t)b * NN;
-  const bool single = (M == 1);
+  float* I = d.intensity + ((size_t)b * M + m) * NN;  // this mode's term (model_intensity sums them)
   pl.forward(
       [&](int y, int x) {
         const float2 v = psi_s[y * N + x];
@@ -318,8 +375,7 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d) {
       [&](int py, int px, float2& v) {
         const int e = py * N + px;
         F[e] = v;
-        if (single) I[e] = abs2f(v) * inv_nn;
-        else atomic_add_f32(&I[e], abs2f(v) * inv_nn);
+        I[e] = abs2f(v) * inv_nn;
       });
 }
 
@@ -374,7 +430,6 @@ __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int cal
   const int M = d.n_modes;
   const int p = d.group_pos[b];
   const float2* F = (const float2*)d.farfield + ((size_t)b * M + m) * NN;
-  const float* I = d.intensity + (size_t)b * NN;
   const float* D = d.patterns_s + (size_t)p * NN;
   const float* mask = d.mask_s;
   const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
@@ -387,7 +442,7 @@ __global__ void __launch_bounds__(T) farfield_update_k(phx_group_desc d, int cal
   pl.inverse(
       [&](int py, int px) {
         const int e = py * N + px;
-        const float Iv = I[e], Dv = D[e], mk = mask[e];
+        const float Iv = model_intensity(d, b, e), Dv = D[e], mk = mask[e];
         const float2 Fv = F[e];
         float t;
         const float u = farfield_factor<MODE>(Iv, Dv, mk, off, eps, var, gscale, t);
@@ -565,15 +620,12 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
   }
   pl.forward_regs(x);
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
-  float* I = d.intensity + (size_t)b * NN;
-  const bool single = M == 1;
+  float* I = d.intensity + ((size_t)b * M + m) * NN;  // this mode's term (model_intensity sums them)
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int e = (pl.QL * k + pl.q) * N + pl.rc;
     F[e] = x[k];
-    const float a = abs2f(x[k]) * inv_nn;
-    if (single) I[e] = a;
-    else atomic_add_f32(&I[e], a);
+    I[e] = abs2f(x[k]) * inv_nn;
   }
   const double tot = block_sum_f64(nrm, red_lds_);
   if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(S, d.bmax, M).norm + b], tot);
@@ -639,14 +691,16 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   float2 x[E];
   // far field: chi_hat = (sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask F
   const float2* F = (const float2*)d.farfield + ((size_t)b * M + m) * NN;
-  const float* I = d.intensity + (size_t)b * NN;
   const float* D = d.patterns_s + (size_t)p * NN;
   const float* mask = d.mask_s;
   const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
   float e2 = 0.f;
+  float Ik[E];
+  model_intensity_cols<N>(d, b, pl.q, pl.rc, Ik);
+  int kk = 0;  // register index of the point cols_apply passes (its loop is unrolled)
   pl.cols_apply(x, [&](int py, int px, float2) {
     const int e = py * N + px;
-    const float Iv = I[e], Dv = D[e], mk = mask[e];
+    const float Iv = Ik[kk++], Dv = D[e], mk = mask[e];
     const float2 Fv = F[e];
     const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
     const float df = Iv - Dv;
@@ -773,15 +827,17 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     pl.rows_apply(x, [&](int yy, int xx, float2) { return cscale(g0[yy * N + xx], (float)NN); }, [] {});
   } else {
     const float2* F = (const float2*)d.farfield + ((size_t)b * M + m) * NN;
-    const float* I = d.intensity + (size_t)b * NN;
     const float* D = d.patterns_s + (size_t)p * NN;
     const float* mask = d.mask_s;
     const float off = d.params[P_OFFSET], eps = d.params[P_EPS], var = d.params[P_VAR];
     const float gscale = d.params[P_LOSS_SCALE];
     float e2 = 0.f;
+    float Ik[E];
+    model_intensity_cols<N>(d, b, pl.q, pl.rc, Ik);
+    int kk = 0;  // register index of the point cols_apply passes (its loop is unrolled)
     pl.cols_apply(x, [&](int py, int px, float2) {
       const int e = py * N + px;
-      const float Iv = I[e], Dv = D[e], mk = mask[e];
+      const float Iv = Ik[kk++], Dv = D[e], mk = mask[e];
       const float2 Fv = F[e];
       float t;
       const float u = farfield_factor<FF>(Iv, Dv, mk, off, eps, var, gscale, t);
@@ -867,11 +923,13 @@ constexpr int UPD_LIST = 256;  // positions listed per pass
 // gradient: for the pixel (y, x) of this thread in the 64 x 4 tile at (ty0, tx0) of
 // slice i, dO = sum_{b,m} chi_i conj(psi_i) and (with WANT_G) g = sum_{b,m} |psi_i|^2
 // over the group's cutouts that cover it, positions in group order, modes in order.
+// PER_POS_W (ePIE) weights position b's term by 1 / aux[bmax + b] (its max |psi|^2).
 // Every thread of the block must call it (it synchronises the block).
-template <bool WANT_G>
+template <bool WANT_G, bool PER_POS_W = false>
 __device__ __forceinline__ void gather_tile(const phx_group_desc& d, int i, int ty0, int tx0, int y, int x,
                                             float2& dO, float& g) {
   __shared__ int s_b[UPD_LIST], s_y0[UPD_LIST], s_x0[UPD_LIST];
+  __shared__ float s_w[PER_POS_W ? UPD_LIST : 1];
   __shared__ int s_cnt;
   const int gs = *d.group_size;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -901,6 +959,7 @@ __device__ __forceinline__ void gather_tile(const phx_group_desc& d, int i, int 
           s_b[r] = b;
           s_y0[r] = y0;
           s_x0[r] = x0;
+          if constexpr (PER_POS_W) s_w[r] = 1.0f / d.aux[d.bmax + b];
         }
         cnt += __popcll(msk);
       }
@@ -941,13 +1000,14 @@ __device__ __forceinline__ void gather_tile(const phx_group_desc& d, int i, int 
         dOb = cadd(dOb, cmulc(cv, ps));
         if (WANT_G) gb += abs2f(ps);
       }
-      dO = cadd(dO, cscale(dOb, w));
+      if constexpr (PER_POS_W) dO = cadd(dO, cscale(dOb, w * s_w[t]));
+      else dO = cadd(dO, cscale(dOb, w));
       if (WANT_G) g += gb * w;
     }
   }
 }
 
-// Accumulators a group step starts from zero (gom, intensity, red, errors); cleared by
+// Accumulators a group step starts from zero (gom, red, errors); cleared by
 // zero4_k, or by the previous group's object update in the same launch.
 struct ClearSpan {
   float* a;
@@ -1297,20 +1357,16 @@ __global__ void __launch_bounds__(T) epie_backward_k(phx_group_desc d, int i, in
   const size_t HW = (size_t)d.obj_h * d.obj_w;
   const float2* chi = chi_in_buf(d, i) + ((size_t)b * M + m) * NN;
   float2* chi_o = chi_out_buf(d, i) + ((size_t)b * M + m) * NN;
-  const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
   const float2* obj_i = (const float2*)d.object + i * HW;
-  float2* dO = (float2*)d.d_obj;
-  const float maxO = d.aux[b], maxPsi = d.aux[bmax + b];
-  const float betaS = d.params[P_BETA_OBJ] / (float)d.n_slices;
+  const float maxO = d.aux[b];
+  // (the object update of slice i is gathered afterwards from chi_i and psi_i by
+  // epie_object_update_k; upd_obj needs nothing from this kernel)
+  (void)upd_obj;
 
   auto src = [&](int y, int x) -> float2 {
     const float2 cv = chi[y * N + x];
     const bool inb = c.inb(y, x);
     const float2 o = inb ? obj_i[c.idx(y, x)] : make_float2(1.f, 0.f);
-    if (upd_obj && inb) {
-      const float2 go = cscale(cscale(cmulc(cv, psi_i[y * N + x]), betaS), 1.0f / maxPsi);
-      atomic_add_c64(&dO[c.idx(y, x)], go);
-    }
     return cscale(cmulc(cv, o), 1.0f / maxO);
   };
   if (i > 0) {
@@ -1325,23 +1381,29 @@ __global__ void __launch_bounds__(T) epie_backward_k(phx_group_desc d, int i, in
     pl.forward(src, [&](int py, int px, float2& v) {
       Fb[py * N + px] = cmulc(v, shift_filter(sy, sx, d.ky_s[py], d.kx_s[px]));
     });
-  } else {
-    for (int e = threadIdx.x; e < NN; e += T) (void)src(e / N, e % N);
   }
 }
 
-__global__ void __launch_bounds__(256) epie_object_apply_k(phx_group_desc d, int i) {
-  const size_t HW = (size_t)d.obj_h * d.obj_w;
-  float2* obj_i = (float2*)d.object + i * HW;
-  float2* dO = (float2*)d.d_obj;
-  bool bad = false;
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < HW; e += (size_t)gridDim.x * 256) {
-    const float2 o = cadd(obj_i[e], dO[e]);
-    obj_i[e] = o;
-    dO[e] = make_float2(0.f, 0.f);
-    bad |= !(isfinite(o.x) && isfinite(o.y));
-  }
-  if (bad) atomicMin(d.finite_flag, d.group_index);
+// ePIE object update of slice i (solvers.py:466-477):
+//   O_i += beta_O / S * sum_b (sum_m chi_i conj(psi_i)) / max_pixels(sum_m |psi_i|^2)_b
+// over the group's cutouts, gathered per 64 x 4 tile from the chi_i and psi_i planes in
+// group order (gather_tile with the per-position weights epie_maxima_k left in aux):
+// no atomics, so the object does not depend on scheduling.  Runs after epie_backward_k
+// of slice i and before that of slice i-1 (which overwrites chi_i's buffer).  Pixels no
+// cutout covers are left untouched.  Grid (ceil(W/64), ceil(H/4)).
+__global__ void __launch_bounds__(256) epie_object_update_k(phx_group_desc d, int i) {
+  const int H = d.obj_h, W = d.obj_w;
+  const int ty0 = blockIdx.y * UPD_TH, tx0 = blockIdx.x * UPD_TW;
+  const int y = ty0 + threadIdx.x / UPD_TW, x = tx0 + threadIdx.x % UPD_TW;
+  float2 dO;
+  float g;
+  gather_tile<false, true>(d, i, ty0, tx0, y, x, dO, g);
+  if (y >= H || x >= W || (dO.x == 0.f && dO.y == 0.f)) return;
+  const float betaS = d.params[P_BETA_OBJ] / (float)d.n_slices;
+  float2* obj_i = (float2*)d.object + (size_t)i * H * W;
+  const float2 o = cadd(obj_i[(size_t)y * W + x], cscale(dO, betaS));
+  obj_i[(size_t)y * W + x] = o;
+  if (!(isfinite(o.x) && isfinite(o.y))) atomicMin(d.finite_flag, d.group_index);
 }
 
 template <int N, int T>
@@ -1541,11 +1603,10 @@ __global__ void __launch_bounds__(256) dry_run_stats_intensity_k(phx_group_desc 
   const int b = blockIdx.x;
   if (!slot_active(d, b)) return;
   const int NN = d.n * d.n;
-  const float* I = d.intensity + (size_t)b * NN;
   const float* D = d.patterns_s + (size_t)d.group_pos[b] * NN;
   double model = 0.0, ex = 0.0;
   for (int e = threadIdx.x; e < NN; e += 256) {
-    model += (double)I[e];
+    model += (double)model_intensity(d, b, e);
     ex += (double)D[e];
   }
   model = block_sum_f64(model, scratch);
@@ -1562,7 +1623,7 @@ __global__ void exit_intensity_copy_k(phx_group_desc d, float* out) {
   if (e >= N * N) return;
   const int py = e / N, px = e % N;
   const int fe = scr_to_freq<N>(py) * N + scr_to_freq<N>(px);
-  out[(size_t)d.group_pos[b] * N * N + fe] = d.intensity[(size_t)b * N * N + e];
+  out[(size_t)d.group_pos[b] * N * N + fe] = model_intensity(d, b, e);
 }
 
 // pos_update[p] = num[b] / den (solvers.py:280), scattered to the group's positions.
@@ -1577,14 +1638,17 @@ __global__ void position_finish_k(phx_group_desc d, float* pos_update) {
 
 __global__ void zero4_k(ClearSpan z) { clear_span(z); }
 
+#include "phx_split256.h"
+
 // --------------------------------- host side ---------------------------------------
 // The accumulators phx_group_begin clears for group descriptor d.
 ClearSpan begin_span(const phx_group_desc& d) {
   ClearSpan z;
   z.a = d.gom;
   z.na = (int64_t)d.n * d.n;
-  z.b = d.intensity;
-  z.nb = (int64_t)d.bmax * d.n * d.n;
+  // intensity needs no clear: the forward kernels store every mode term of an active slot
+  z.b = nullptr;
+  z.nb = 0;
   z.c = d.red;
   z.nc = red_layout(d.n_slices, d.bmax, d.n_modes).total;
   z.e = d.errors;
@@ -1592,7 +1656,7 @@ ClearSpan begin_span(const phx_group_desc& d) {
   return z;
 }
 
-bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192; }
+bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256; }
 
 int check_desc(const phx_group_desc* d) {
   if (!d || !supported_n(d->n) || d->n_modes < 1 || d->n_slices < 1 || d->bmax < 1 || d->obj_h < 1 ||
@@ -1706,6 +1770,7 @@ int phx_probe_spectra(const phx_group_desc* d, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
   if (!d->probe_k) return PHX_EINVAL;
+  if (d->n == S256) return s256_probe_spectra(*d, stream);
 #define L(N, T) hipLaunchKernelGGL((probe_spectra_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -1728,6 +1793,7 @@ int phx_shift_probes(const phx_group_desc* d, int illum, int propagate, hipStrea
   if (r) return r;
   if (!d->probe_k || (illum && !d->gpm)) return PHX_EINVAL;
   if (propagate && d->n_slices < 2) return PHX_EINVAL;
+  if (d->n == S256) return illum ? PHX_EUNSUPPORTED : s256_shift_probes(*d, propagate, stream);
 #define L(N, T)                                                                                          \
   do {                                                                                                   \
     if (illum)                                                                                           \
@@ -1745,6 +1811,7 @@ int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t
   int r = check_desc(d);
   if (r) return r;
   if (slice < 0 || slice > d->n_slices - 2 || (illum && !d->gpm)) return PHX_EINVAL;
+  if (d->n == S256) return illum ? PHX_EUNSUPPORTED : s256_forward_slice(*d, slice, stream);
 #define L(N, T)                                                                                          \
   do {                                                                                                   \
     if (illum)                                                                                           \
@@ -1764,6 +1831,7 @@ int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t
   if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a || (illum && !d->gpm) ||
       (calc_error && !d->errors))
     return PHX_EINVAL;
+  if (d->n == S256) return illum ? PHX_EUNSUPPORTED : s256_farfield(*d, calc_error, stream);
 #define L(N, T)                                                                                          \
   do {                                                                                                   \
     if (illum)                                                                                           \
@@ -1789,6 +1857,7 @@ int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, in
   if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (update_object && !d->d_obj) ||
       (update_probe && !d->d_probe))
     return PHX_EINVAL;
+  if (d->n == S256) return s256_lsqml_backward(*d, slice, update_object, update_probe, stream);
 #define L(N, T) launch_lsqml_backward<N, T>(*d, slice, update_object, update_probe, stream)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -2013,6 +2082,7 @@ int phx_lsqml_probe_apply(const phx_group_desc* d, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
   if (!d->d_probe || !d->gom || !d->finite_flag) return PHX_EINVAL;
+  if (d->n == S256) return s256_lsqml_probe_apply(*d, stream);
 #define L(N, T) hipLaunchKernelGGL((lsqml_probe_apply_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -2065,7 +2135,7 @@ int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int
                       hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || !d->aux || (update_object && !d->d_obj) ||
+  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || !d->aux ||
       (update_probe && (!d->d_probe || !d->farfield)))
     return PHX_EINVAL;
   hipLaunchKernelGGL(epie_maxima_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d, slice);
@@ -2093,11 +2163,9 @@ int phx_probe_limit_support(const phx_group_desc* d, const float* mask_s, hipStr
 int phx_epie_object_apply(const phx_group_desc* d, int slice, hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (slice < 0 || slice >= d->n_slices || !d->d_obj || !d->finite_flag) return PHX_EINVAL;
-  const int64_t HW = (int64_t)d->obj_h * d->obj_w;
-  int64_t blocks = (HW + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(epie_object_apply_k, dim3((unsigned)blocks), dim3(256), 0, stream, *d, slice);
+  if (slice < 0 || slice >= d->n_slices || !d->finite_flag || !d->aux || !d->chi_a || !d->chi_b) return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->obj_w + UPD_TW - 1) / UPD_TW), (unsigned)((d->obj_h + UPD_TH - 1) / UPD_TH));
+  hipLaunchKernelGGL(epie_object_update_k, grid, dim3(256), 0, stream, *d, slice);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }
@@ -2213,11 +2281,18 @@ int phx_exit_intensity(const phx_group_desc* d, float* out, hipStream_t stream) 
   int r = check_desc(d);
   if (r) return r;
   if (!out || !d->farfield || !d->intensity) return PHX_EINVAL;
+  const dim3 grid((unsigned)((d->n * d->n + 255) / 256), (unsigned)d->bmax);
+  if (d->n == S256) {
+    r = s256_exit_spectra(*d, stream);
+    if (r) return r;
+    hipLaunchKernelGGL((exit_intensity_copy_k<S256>), grid, dim3(256), 0, stream, *d, out);
+    PHX_CHECK_LAUNCH();
+    return PHX_OK;
+  }
 #define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T, false>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
   PHX_CHECK_LAUNCH();
-  const dim3 grid((unsigned)((d->n * d->n + 255) / 256), (unsigned)d->bmax);
 #define L(N, T) hipLaunchKernelGGL((exit_intensity_copy_k<N>), grid, dim3(256), 0, stream, *d, out)
   PHX_DISPATCH(d->n, L);
 #undef L

@@ -218,6 +218,7 @@ struct PlaneLanes {
 // (N = 128): register k = 2 r + s of lane q holds X[2 q + s + 16 r].
 template <int N>
 __host__ __device__ __forceinline__ int scr_to_freq(int p) {
+  if (N == 256) return p;  // 256 x 256 planes (split over launches) keep natural order
   constexpr int L = PlaneLanes<N>::value;
   const int q = p % L, k = p / L;
   if (L == 8) return 2 * q + (k & 1) + 16 * (k >> 1);

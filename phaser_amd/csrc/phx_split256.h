@@ -1,0 +1,480 @@
+// Group kernels for 256 x 256 planes (included by phx_engine.hip inside its anonymous
+// namespace, after the group helpers it uses: Cut, obj_at, shift_filter, prop_of,
+// psi_off, chi_in_buf/chi_out_buf, model_intensity, farfield_factor, red_layout).
+//
+// A 256 x 256 complex64 plane (512 KiB) fits neither a CU's LDS nor one workgroup's
+// registers, so every 2-D transform of a group step is split over launches: a row pass
+// (workgroups of 16 whole rows, 256 threads) and a column pass (workgroups of 32 whole
+// columns, 512 threads), the plane going through HBM between them.  Element-wise work is
+// applied where a pass loads or stores a point (the cutout and transmission before a
+// forward row transform, the propagator between a column pass's forward and inverse
+// transforms, the modulus constraint before an inverse column transform), so a slice step
+// of the forward model is three passes over its plane (rows, columns, rows) and each pass
+// reads and writes the plane once.  The wave in flight between passes lives in the
+// buffer that receives the step's result (psi[i+1], chi_{i-1}, farfield), so no scratch
+// plane is needed.
+//
+// Layout: every plane in natural order, spectra included (frequency index k at k, the
+// host's ky/kx tables and the patterns in that order).  Transforms are raw (unnormalised)
+// DFTs, W = exp(-2 pi i / 256) forward; the 1/n^2 of an inverse 2-D transform is applied
+// where its row pass stores the result, as the single-workgroup plane kernels do.
+//
+// One 256-point line is held by 16 threads: thread t of a line holds x[16 a + t] in
+// register a, and the transform is the four-step 16 x 16 split
+//   X[k1 + 16 k2] = sum_t W_16^(t k2) W_256^(t k1) sum_a x[16 a + t] W_16^(a k1):
+// an in-register DFT_16, the twiddle W_256^(t k1), an exchange through LDS after which
+// thread t holds the 16 values of k1 = t, and a second in-register DFT_16 that leaves
+// X[t + 16 k2] in register k2.  The output index of register k equals the input index
+// of register a = k, so a forward transform, a point-wise product and an inverse
+// transform chain in registers.
+
+constexpr int S256 = 256;
+constexpr int S256_NN = S256 * S256;
+constexpr int S256_LD = 17;                    // padded row of a line's 16 x 16 exchange block
+constexpr int S256_ROW_LINES = 16;             // rows per row-pass workgroup (256 threads)
+constexpr int S256_COL_LINES = 32;             // columns per column-pass workgroup (512 threads)
+constexpr int S256_ROW_TILES = S256 / S256_ROW_LINES;
+constexpr int S256_COL_TILES = S256 / S256_COL_LINES;
+constexpr float S256_INV_NN = 1.0f / (float)S256_NN;
+
+// Exchange blocks and the twiddle table of one pass workgroup.  In the column pass the
+// 32 lines of a load are adjacent lanes, so the blocks sit at an odd stride.
+template <bool COL>
+struct S256Lds {
+  static constexpr int LINES = COL ? S256_COL_LINES : S256_ROW_LINES;
+  static constexpr int BLK = 16 * S256_LD + (COL ? 1 : 0);
+  float2 ex[LINES * BLK];
+  float2 tw[S256];
+  double red[COL ? 8 : 4];  // one FP64 partial per wave for block sums
+};
+
+// Pass geometry: line l of the workgroup, thread t within the line, the line's index in
+// the plane, and the plane element held in register a.
+template <bool COL>
+struct S256Pass {
+  int l, t, line;
+  __device__ __forceinline__ S256Pass(int tile) {
+    const int tid = threadIdx.x;
+    l = COL ? (tid % S256_COL_LINES) : (tid / 16);
+    t = COL ? (tid / S256_COL_LINES) : (tid % 16);
+    line = (COL ? S256_COL_LINES : S256_ROW_LINES) * tile + l;
+  }
+  // (row, column) of register a
+  __device__ __forceinline__ int y(int a) const { return COL ? 16 * a + t : line; }
+  __device__ __forceinline__ int x(int a) const { return COL ? line : 16 * a + t; }
+  __device__ __forceinline__ int e(int a) const { return y(a) * S256 + x(a); }
+};
+
+template <bool COL>
+__device__ __forceinline__ void s256_load_twiddles(S256Lds<COL>& s, const phx_group_desc& d) {
+  constexpr int T = 16 * S256Lds<COL>::LINES;
+  for (int k = threadIdx.x; k < S256; k += T) s.tw[k] = ((const float2*)d.twiddle)[k];
+  __syncthreads();
+}
+
+// One 256-point transform of every line of the workgroup (see the header comment).  All
+// threads of the block call it (it synchronises the block).
+template <bool COL, bool INV>
+__device__ __forceinline__ void s256_line(float2 (&x)[16], S256Lds<COL>& s, int l, int t) {
+  dft<16, INV>(x);
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) {
+    const float2 w = s.tw[(t * k1) & 255];
+    x[k1] = INV ? cmulc(x[k1], w) : cmul(x[k1], w);
+  }
+  float2* blk = s.ex + l * S256Lds<COL>::BLK;
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) blk[k1 * S256_LD + t] = x[k1];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) x[j] = blk[t * S256_LD + j];
+  __syncthreads();  // the block is free for the next transform
+  dft<16, INV>(x);
+}
+
+// FP64 sum over the block of a per-thread FP32 partial (all threads call it).
+template <bool COL>
+__device__ __forceinline__ double s256_block_sum(float v, S256Lds<COL>& s) {
+  return block_sum_f64((double)v, s.red);
+}
+
+// ----- operations of the passes -----
+enum S256Op {
+  // row passes
+  R_PROBE_FWD = 0,   // probe[m] -> forward rows -> probe_k[m]                        (per mode)
+  R_SHIFT_FINISH,    // psi_0 semi -> inverse rows / n^2 -> psi_0, norm; [* O_0 -> fwd rows -> psi_1]
+  R_SLICE_START,     // psi_i * O_i -> forward rows -> psi_{i+1}
+  R_WAVE_FINISH,     // psi_{i+1} semi -> inverse rows / n^2 -> psi_{i+1}
+  R_EXIT,            // psi_{S-1} * O_{S-1} -> forward rows -> farfield
+  R_CHI_FINISH,      // chi_a semi -> inverse rows / n^2 -> chi_a
+  R_BACK,            // chi_i: step-size sums, dP = chi conj(O_i) -> forward rows -> chi_{i-1} or farfield
+  R_BACK_FINISH,     // chi_{i-1} semi -> inverse rows / n^2 -> chi_{i-1}
+  R_PROBE_UPDATE,    // d_probe semi -> inverse rows, P += update; P -> forward rows -> d_probe (per mode)
+  // column passes
+  C_PROBE_FWD,       // probe_k[m] forward columns in place                          (per mode)
+  C_SHIFT,           // probe_k[m] filt_b -> inverse columns -> psi_0
+  C_PROPAGATE,       // forward columns, * p_i (or conj(p_{i-1}) backward), inverse columns, in place
+  C_EXIT,            // forward columns -> farfield, intensity[b][m] = |F|^2 / n^2
+  C_MODULUS,         // modulus update of farfield -> inverse columns -> chi_a
+  C_BACK_PROBE,      // forward columns of dP, * conj(filt_b) -> farfield (per-position probe spectrum)
+  C_PROBE_INV,       // d_probe[m] inverse columns in place                          (per mode)
+  C_PROBE_SPEC,      // d_probe[m] forward columns -> probe_k[m]; d_probe = 0        (per mode)
+};
+
+// Row pass.  Grid (S256_ROW_TILES * M, bmax), or (S256_ROW_TILES * M, 1) for the
+// per-mode operations; 256 threads.  `i` is the slice; flags: bit 0 propagate
+// (R_SHIFT_FINISH) / object sums (R_BACK), bit 1 probe sums and spectrum (R_BACK).
+template <int OP>
+__global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int flags) {
+  __shared__ S256Lds<false> s;
+  const int tile = blockIdx.x % S256_ROW_TILES, m = blockIdx.x / S256_ROW_TILES, b = blockIdx.y;
+  constexpr bool PER_MODE = OP == R_PROBE_FWD || OP == R_PROBE_UPDATE;
+  if (!PER_MODE && !slot_active(d, b)) return;
+  const S256Pass<false> g(tile);
+  s256_load_twiddles<false>(s, d);
+  const int M = d.n_modes, bmax = d.bmax;
+  const size_t plane = ((size_t)b * M + m) * S256_NN;
+  const size_t HW = (size_t)d.obj_h * d.obj_w;
+  float2 x[16];
+
+  if constexpr (OP == R_PROBE_FWD) {
+    const float2* P = (const float2*)d.probe + (size_t)m * S256_NN;
+    float2* Pk = (float2*)d.probe_k + (size_t)m * S256_NN;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = P[g.e(a)];
+    s256_line<false, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) Pk[g.e(a)] = x[a];
+  } else if constexpr (OP == R_SHIFT_FINISH || OP == R_WAVE_FINISH || OP == R_CHI_FINISH || OP == R_BACK_FINISH) {
+    // the inverse row transform that completes an inverse 2-D transform, stored / n^2
+    float2* w;
+    if (OP == R_SHIFT_FINISH) w = (float2*)d.psi + psi_off(d, 0, b, m);
+    else if (OP == R_WAVE_FINISH) w = (float2*)d.psi + psi_off(d, i + 1, b, m);
+    else if (OP == R_CHI_FINISH) w = (float2*)d.chi_a + plane;
+    else w = chi_out_buf(d, i) + plane;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = w[g.e(a)];
+    s256_line<false, true>(x, s, g.l, g.t);
+    float nrm = 0.f;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      x[a] = cscale(x[a], S256_INV_NN);
+      w[g.e(a)] = x[a];
+      if (OP == R_SHIFT_FINISH) nrm += abs2f(x[a]);
+    }
+    if constexpr (OP == R_SHIFT_FINISH) {
+      // norm[b] = sum_m |psi_0|^2 (FP64), then the slice-0 step's first pass
+      const double tot = s256_block_sum<false>(nrm, s);
+      if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(d.n_slices, bmax, M).norm + b], tot);
+      if (flags & 1) {
+        const Cut c = cut_of(d, d.group_pos[b]);
+        const float2* obj0 = (const float2*)d.object;
+        float2* psi1 = (float2*)d.psi + psi_off(d, 1, b, m);
+#pragma unroll
+        for (int a = 0; a < 16; ++a) x[a] = cmul(x[a], obj_at(obj0, c, g.y(a), g.x(a)));
+        s256_line<false, false>(x, s, g.l, g.t);
+#pragma unroll
+        for (int a = 0; a < 16; ++a) psi1[g.e(a)] = x[a];
+      }
+    }
+  } else if constexpr (OP == R_SLICE_START || OP == R_EXIT) {
+    const int si = OP == R_EXIT ? d.n_slices - 1 : i;
+    const Cut c = cut_of(d, d.group_pos[b]);
+    const float2* psi_i = (const float2*)d.psi + psi_off(d, si, b, m);
+    const float2* obj_i = (const float2*)d.object + si * HW;
+    float2* out = OP == R_EXIT ? (float2*)d.farfield + plane : (float2*)d.psi + psi_off(d, si + 1, b, m);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = cmul(psi_i[g.e(a)], obj_at(obj_i, c, g.y(a), g.x(a)));
+    s256_line<false, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) out[g.e(a)] = x[a];
+  } else if constexpr (OP == R_BACK) {
+    // LSQ-ML backward step at slice i (solvers.py:242-271), its first pass: the step-size
+    // sums of the (pre-update) cutouts and dP = chi conj(O_i), transformed along rows into
+    // chi_{i-1}'s buffer (i > 0) or the farfield slot (i == 0 with a probe update)
+    const bool upd_obj = flags & 1, probe = flags & 2;
+    const Cut c = cut_of(d, d.group_pos[b]);
+    const float2* chi = chi_in_buf(d, i) + plane;
+    const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
+    const float2* obj_i = (const float2*)d.object + i * HW;
+    float numO = 0.f, denO = 0.f, numP = 0.f, denP = 0.f;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const float2 cv = chi[g.e(a)];
+      const float2 o = obj_at(obj_i, c, g.y(a), g.x(a));
+      if (upd_obj) {
+        const float2 ps = psi_i[g.e(a)];
+        const float2 tt = cmul(cmulc(cv, ps), ps);  // (chi conj(psi)) psi
+        numO += cv.x * tt.x + cv.y * tt.y;
+        denO += abs2f(tt);
+      }
+      const float2 dP = cmulc(cv, o);
+      if (probe) {
+        const float2 t2 = cmul(dP, o);
+        numP += cv.x * t2.x + cv.y * t2.y;
+        denP += abs2f(t2);
+      }
+      x[a] = dP;
+    }
+    const RedLayout L = red_layout(d.n_slices, bmax, M);
+    if (upd_obj) {
+      const double ta = s256_block_sum<false>(numO, s), tb = s256_block_sum<false>(denO, s);
+      if (threadIdx.x == 0) {
+        atomic_add_f64(&d.red[L.num_obj + i * bmax + b], ta);
+        atomic_add_f64(&d.red[L.den_obj + i], tb);
+      }
+    }
+    if (probe) {
+      const double ta = s256_block_sum<false>(numP, s), tb = s256_block_sum<false>(denP, s);
+      if (threadIdx.x == 0) {
+        atomic_add_f64(&d.red[L.numP + b * M + m], ta);
+        atomic_add_f64(&d.red[L.denP], tb);
+      }
+    }
+    if (i > 0 || probe) {
+      float2* out = i > 0 ? chi_out_buf(d, i) + plane : (float2*)d.farfield + plane;
+      s256_line<false, false>(x, s, g.l, g.t);
+#pragma unroll
+      for (int a = 0; a < 16; ++a) out[g.e(a)] = x[a];
+    }
+  } else if constexpr (OP == R_PROBE_UPDATE) {
+    // LSQ-ML probe step (solvers.py:262-270), its row pass: P += beta_P alpha_P
+    // avg/(gom + reg) * gom/(gom + 1e-16) with avg = ifft2(d_probe), then the forward
+    // row transform of the new probe into d_probe (the column pass finishes probe_k)
+    __shared__ float sA;
+    if (threadIdx.x < 64) {
+      const RedLayout L = red_layout(d.n_slices, bmax, M);
+      const int gs = *d.group_size;
+      double sacc = 0.0;
+      for (int bb = threadIdx.x; bb < gs; bb += 64) sacc += d.red[L.numP + bb * M + m];
+      sacc = wave_sum_f64(sacc);
+      if (threadIdx.x == 0) {
+        const double den = d.red[L.denP] + (double)d.params[P_GAMMA];
+        sA = (float)((double)d.params[P_BETA_PROBE] * (sacc / den));
+      }
+    }
+    float2* dPk = (float2*)d.d_probe + (size_t)m * S256_NN;
+    float2* P = (float2*)d.probe + (size_t)m * S256_NN;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = dPk[g.e(a)];
+    s256_line<false, true>(x, s, g.l, g.t);  // its barriers publish sA
+    const float A = sA, reg = d.params[P_REG_PROBE];
+    bool bad = false;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int e = g.e(a);
+      const float gm = d.gom[e];
+      const float2 avg = cscale(cscale(x[a], S256_INV_NN), 1.0f / (gm + reg));
+      const float2 u = cscale(cscale(cscale(avg, A), gm), 1.0f / (gm + 1e-16f));
+      const float2 o = cadd(P[e], u);
+      P[e] = o;
+      bad |= !(isfinite(o.x) && isfinite(o.y));
+      x[a] = o;
+    }
+    if (bad) atomicMin(d.finite_flag, d.group_index);
+    s256_line<false, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) dPk[g.e(a)] = x[a];
+  }
+}
+
+// Column pass.  Grid (S256_COL_TILES * M, bmax), or (S256_COL_TILES * M, 1) for the
+// per-mode operations; 512 threads.  C_PROPAGATE: `i` is the propagator index and flags
+// bit 0 selects conj(p) (back-propagation); the plane is psi[i+1] (forward) or
+// chi_out_buf(i + 1) (backward).  C_EXIT / C_MODULUS: flags bit 0 marks the last slice.
+template <int OP>
+__global__ void __launch_bounds__(512) s256_cols_k(phx_group_desc d, int i, int flags) {
+  __shared__ S256Lds<true> s;
+  const int tile = blockIdx.x % S256_COL_TILES, m = blockIdx.x / S256_COL_TILES, b = blockIdx.y;
+  constexpr bool PER_MODE = OP == C_PROBE_FWD || OP == C_PROBE_INV || OP == C_PROBE_SPEC;
+  if (!PER_MODE && !slot_active(d, b)) return;
+  const S256Pass<true> g(tile);
+  s256_load_twiddles<true>(s, d);
+  const int M = d.n_modes;
+  const size_t plane = ((size_t)b * M + m) * S256_NN;
+  float2 x[16];
+
+  if constexpr (OP == C_PROBE_FWD || OP == C_PROBE_INV) {
+    float2* w = OP == C_PROBE_FWD ? (float2*)d.probe_k + (size_t)m * S256_NN : (float2*)d.d_probe + (size_t)m * S256_NN;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = w[g.e(a)];
+    s256_line<true, OP == C_PROBE_INV>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) w[g.e(a)] = x[a];
+  } else if constexpr (OP == C_PROBE_SPEC) {
+    float2* w = (float2*)d.d_probe + (size_t)m * S256_NN;
+    float2* Pk = (float2*)d.probe_k + (size_t)m * S256_NN;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = w[g.e(a)];
+#pragma unroll
+    for (int a = 0; a < 16; ++a) w[g.e(a)] = make_float2(0.f, 0.f);  // d_probe is kept zero
+    s256_line<true, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) Pk[g.e(a)] = x[a];
+  } else if constexpr (OP == C_SHIFT) {
+    // psi_0 = ifft2(fft2(P_m) filt_b) (simulation.py:241-260), its inverse column pass
+    const int p = d.group_pos[b];
+    const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+    const float2* Pk = (const float2*)d.probe_k + (size_t)m * S256_NN;
+    float2* psi0 = (float2*)d.psi + psi_off(d, 0, b, m);
+#pragma unroll
+    for (int a = 0; a < 16; ++a)
+      x[a] = cmul(Pk[g.e(a)], shift_filter(sy, sx, d.ky_s[g.y(a)], d.kx_s[g.x(a)]));
+    s256_line<true, true>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) psi0[g.e(a)] = x[a];
+  } else if constexpr (OP == C_PROPAGATE) {
+    const bool back = flags & 1;
+    float2* w = back ? chi_out_buf(d, i + 1) + plane : (float2*)d.psi + psi_off(d, i + 1, b, m);
+    const float2* prop = prop_of(d, i, b);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = w[g.e(a)];
+    s256_line<true, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const float2 pv = prop[g.e(a)];
+      x[a] = back ? cmulc(x[a], pv) : cmul(x[a], pv);
+    }
+    s256_line<true, true>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) w[g.e(a)] = x[a];
+  } else if constexpr (OP == C_EXIT) {
+    float2* F = (float2*)d.farfield + plane;
+    float* I = d.intensity + plane;  // this mode's term (model_intensity sums them)
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = F[g.e(a)];
+    s256_line<true, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      F[g.e(a)] = x[a];
+      I[g.e(a)] = abs2f(x[a]) * S256_INV_NN;
+    }
+  } else if constexpr (OP == C_MODULUS) {
+    // chi_hat = (sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask F (noise_models.py:47-63), then
+    // its inverse column transform into chi_a
+    const float2* F = (const float2*)d.farfield + plane;
+    const float* D = d.patterns_s + (size_t)d.group_pos[b] * S256_NN;
+    const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
+    float2* chi = (float2*)d.chi_a + plane;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int e = g.e(a);
+      float t;
+      const float u = farfield_factor<FM_MODULUS>(model_intensity(d, b, e), D[e], d.mask_s[e], off, eps, 0.f, 1.f, t);
+      x[a] = cscale(F[e], u);
+    }
+    s256_line<true, true>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) chi[g.e(a)] = x[a];
+  } else if constexpr (OP == C_BACK_PROBE) {
+    // per-position probe-update spectrum fft2(dP) conj(filt_b) (solvers.py:262-270), summed
+    // over positions in order by probe_spectrum_reduce_k
+    const int p = d.group_pos[b];
+    const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+    float2* Fp = (float2*)d.farfield + plane;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = Fp[g.e(a)];
+    s256_line<true, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a)
+      Fp[g.e(a)] = cmulc(x[a], shift_filter(sy, sx, d.ky_s[g.y(a)], d.kx_s[g.x(a)]));
+  }
+}
+
+// errors[b] = sqrt(nansum((I - D)^2)) over the slot's pattern (solvers.py:237; unmasked,
+// FP64 block sum).  One workgroup per slot.
+__global__ void __launch_bounds__(256) s256_errors_k(phx_group_desc d) {
+  __shared__ double scratch[4];
+  const int b = blockIdx.x;
+  if (!slot_active(d, b)) return;
+  const float* D = d.patterns_s + (size_t)d.group_pos[b] * S256_NN;
+  double e2 = 0.0;
+  for (int e = threadIdx.x; e < S256_NN; e += 256) {
+    const float df = model_intensity(d, b, e) - D[e];
+    const float t = df * df;
+    e2 += isnan(t) ? 0.0 : (double)t;
+  }
+  e2 = block_sum_f64(e2, scratch);
+  if (threadIdx.x == 0) d.errors[b] = (float)sqrt(e2);
+}
+
+// ----- host launchers (n = 256) -----
+inline dim3 s256_row_grid(const phx_group_desc& d, bool per_mode) {
+  return dim3((unsigned)(S256_ROW_TILES * d.n_modes), per_mode ? 1u : (unsigned)d.bmax);
+}
+inline dim3 s256_col_grid(const phx_group_desc& d, bool per_mode) {
+  return dim3((unsigned)(S256_COL_TILES * d.n_modes), per_mode ? 1u : (unsigned)d.bmax);
+}
+#define S256_ROWS(OP, PM, I, F) \
+  hipLaunchKernelGGL((s256_rows_k<OP>), s256_row_grid(d, PM), dim3(256), 0, stream, d, (I), (F))
+#define S256_COLS(OP, PM, I, F) \
+  hipLaunchKernelGGL((s256_cols_k<OP>), s256_col_grid(d, PM), dim3(512), 0, stream, d, (I), (F))
+
+inline int s256_probe_spectra(const phx_group_desc& d, hipStream_t stream) {
+  S256_ROWS(R_PROBE_FWD, true, 0, 0);
+  S256_COLS(C_PROBE_FWD, true, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+inline int s256_shift_probes(const phx_group_desc& d, int propagate, hipStream_t stream) {
+  S256_COLS(C_SHIFT, false, 0, 0);
+  S256_ROWS(R_SHIFT_FINISH, false, 0, propagate ? 1 : 0);
+  if (propagate) {
+    S256_COLS(C_PROPAGATE, false, 0, 0);
+    S256_ROWS(R_WAVE_FINISH, false, 0, 0);
+  }
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+inline int s256_forward_slice(const phx_group_desc& d, int i, hipStream_t stream) {
+  S256_ROWS(R_SLICE_START, false, i, 0);
+  S256_COLS(C_PROPAGATE, false, i, 0);
+  S256_ROWS(R_WAVE_FINISH, false, i, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// farfield = FFT(psi[S-1] O_{S-1}) and intensity[b][m] = |farfield|^2 / n^2
+inline int s256_exit_spectra(const phx_group_desc& d, hipStream_t stream) {
+  S256_ROWS(R_EXIT, false, 0, 0);
+  S256_COLS(C_EXIT, false, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+inline int s256_farfield(const phx_group_desc& d, int calc_error, hipStream_t stream) {
+  S256_ROWS(R_EXIT, false, 0, 0);
+  S256_COLS(C_EXIT, false, 0, 0);
+  if (calc_error) hipLaunchKernelGGL(s256_errors_k, dim3((unsigned)d.bmax), dim3(256), 0, stream, d);
+  S256_COLS(C_MODULUS, false, 0, 0);
+  S256_ROWS(R_CHI_FINISH, false, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+inline int s256_lsqml_backward(const phx_group_desc& d, int i, int upd_obj, int upd_probe, hipStream_t stream) {
+  const bool probe = i == 0 && upd_probe;
+  S256_ROWS(R_BACK, false, i, (upd_obj ? 1 : 0) | (probe ? 2 : 0));
+  if (i > 0) {
+    // chi_{i-1} = ifft2(fft2(dP) conj(p_{i-1})), in chi_out_buf(i)
+    S256_COLS(C_PROPAGATE, false, i - 1, 1);
+    S256_ROWS(R_BACK_FINISH, false, i, 0);
+  } else if (probe) {
+    S256_COLS(C_BACK_PROBE, false, 0, 0);
+  }
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+inline int s256_lsqml_probe_apply(const phx_group_desc& d, hipStream_t stream) {
+  S256_COLS(C_PROBE_INV, true, 0, 0);
+  S256_ROWS(R_PROBE_UPDATE, true, 0, 0);
+  S256_COLS(C_PROBE_SPEC, true, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+#undef S256_ROWS
+#undef S256_COLS

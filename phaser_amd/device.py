@@ -7,7 +7,7 @@ HBM layout (config with P positions, n x n planes, M modes, S slices, H x W slic
   patterns_s (P, n*n) f32 and mask_s (n*n) f32 are resident, in scrambled frequency order;
   props_s (S-1, n*n) c64 scrambled; psi (S, B, M, n, n) c64 holds every slice's forward
   wave for the backward pass; chi_a / chi_b (B, M, n, n) ping-pong the back-propagated
-  exit-wave update; farfield (B, M, n*n) c64 and intensity (B, n*n) f32 carry the exit
+  exit-wave update; farfield (B, M, n*n) c64 and intensity (B, M, n*n) f32 carry the exit
   spectra between the two far-field kernels; gpm (S, H, W) f32, d_obj (H, W) c64,
   d_probe (M, n*n) c64 are the scatter accumulators; red holds the FP64 step-size sums.
 Per-iteration tables: group_pos (G, B) int32, group_size (G,) int32, start (P, 2) int32,
@@ -24,6 +24,10 @@ from .optics import fft_lanes, scr_to_freq, twiddles
 INT32_MAX = 2**31 - 1
 # largest plane whose LSQ-ML probe step runs in strips (phx_lsqml_probe_apply_strips)
 PROBE_STRIPS_MAX_N = 128
+# plane sides with group kernels; 256 x 256 planes run the LSQ-ML step only, in split
+# passes (row and column launches through HBM), without position correction
+PLANE_SIDES = (16, 32, 64, 128, 192, 256)
+SPLIT_N = 256
 # largest plane side the whole-sweep kernels are used for (larger: one launch per slice)
 FUSED_MAX_N = 128
 # the object-update launch of group g also clears group g+1's accumulators
@@ -122,9 +126,9 @@ class GroupWorkspace:
         assert obj.is_contiguous() and probe.is_contiguous()
         self.S, self.H, self.W = (int(v) for v in obj.shape)
         self.M, ny, nx = (int(v) for v in probe.shape)
-        if ny != nx or ny not in (16, 32, 64, 128, 192):
+        if ny != nx or ny not in PLANE_SIDES:
             raise NotImplementedError(f"probe shape {ny}x{nx}: the plane kernels support square planes of side "
-                                      "16, 32, 64, 128 or 192")
+                                      "16, 32, 64, 128, 192 or 256")
         self.n = n = ny
         self.NN = n * n
         pats = host_patterns(patterns)
@@ -172,7 +176,9 @@ class GroupWorkspace:
         self.chi_a = torch.empty((B, M, NN), dtype=c64, device=dev)
         self.chi_b = torch.empty((B, M, NN), dtype=c64, device=dev)
         self.farfield = torch.empty((B, M, NN), dtype=c64, device=dev)
-        self.intensity = torch.empty((B, NN), dtype=f32, device=dev)
+        # one plane per mode: the forward kernels store |F_m|^2 / n^2 and the readers add
+        # the modes in order (no atomics: the model intensity is the same on every run)
+        self.intensity = torch.empty((B, self.M, NN), dtype=f32, device=dev)
         self.gpm = torch.empty((S, H, W), dtype=f32, device=dev)
         self.d_obj = torch.zeros((H, W), dtype=c64, device=dev)
         self.d_probe = torch.zeros((M, NN), dtype=c64, device=dev)
@@ -373,8 +379,8 @@ class GroupWorkspace:
             # captured successor runs on its queue, later ones on other queues.  The order
             # below keeps the chain begin -> forward -> backward -> update -> next forward
             # on one queue (no cross-queue hop between the update and the next forward) and
-            # the probe-side kernels on a second one.  Group g+1's accumulators (intensity,
-            # its own reduction slot and error row, the object-magnitude plane of its
+            # the probe-side kernels on a second one.  Group g+1's accumulators (its own
+            # reduction slot and error row, the object-magnitude plane of its
             # parity) are free once group g's backward sweep has run, so they are cleared
             # there; the streams join before each forward sweep, which needs the probe
             # spectra of the previous group's probe step, and after the last group.

@@ -21,7 +21,7 @@ import numpy
 
 from .. import _lib
 from ..constraints import CostRegularizers, make_constraints
-from ..device import FUSED_MAX_N, GroupWorkspace, host_patterns, stream_handle, INT32_MAX
+from ..device import FUSED_MAX_N, SPLIT_N, GroupWorkspace, host_patterns, stream_handle, INT32_MAX
 from ..groups import GroupManager
 from ..optics import make_propagators
 from ..plan import process_flag, process_schedule, shuffle_groups_flag
@@ -172,11 +172,18 @@ class MultisliceModel:
         return self.ws.red[g, o:o + 2 * self.ws.B].view(self.ws.B, 2)
 
 
+def _check_plane_side(state):
+    if int(state.probe.data.shape[-1]) == SPLIT_N:
+        raise NotImplementedError(f"{SPLIT_N} x {SPLIT_N} planes: the MI355X gradient engine has no adjoint kernels "
+                                  "at this size (the conventional LSQ-ML engine runs it)")
+
+
 def make_model(state, patterns, pattern_mask, noise, *, groups, bwlim_frac=2 / 3, recompute=False):
     """MultisliceModel over `state` (moved to the GPU if needed) for the given groups
     (list of index arrays into the scan shape).  Returns (model, device state)."""
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
+    _check_plane_side(state)
     props = make_propagators(state.probe.sampling, state.wavelength, state.object.thicknesses, bwlim_frac)
     bmax = max(numpy.asarray(g).shape[-1] for g in groups)
     ws = GroupWorkspace(state, host_patterns(patterns), numpy.asarray(pattern_mask), props,
@@ -357,6 +364,7 @@ def run_engine(args, props, *, recompute=False):
     seed = args.get('seed')
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
+    _check_plane_side(state)
     noise = props.noise_model
     all_vars, group_solvers, iter_solvers = process_solvers(props)
     if 'tilt' in all_vars:

@@ -14,7 +14,7 @@ _E_REST = 5.1099906e5   # eV
 _HC = 1.23984244e4      # eV * angstrom
 
 # plane sides the FFT kernels are built for (phaser_amd/csrc/phx_fft.h)
-FFT_SIZES = (16, 32, 64, 128, 192)
+FFT_SIZES = (16, 32, 64, 128, 192, 256)
 
 
 def electron_wavelength(energy_ev: float) -> float:
@@ -84,8 +84,10 @@ def scr_to_freq(n: int) -> numpy.ndarray:
     holds X[2 q + s + 16 r]."""
     if n not in FFT_SIZES:
         raise ValueError(f"unsupported plane side {n}; the FFT kernels support {FFT_SIZES}")
-    L = fft_lanes(n)
     p = numpy.arange(n)
+    if n == 256:
+        return p   # 256 x 256 planes are transformed in split passes and keep natural order
+    L = fft_lanes(n)
     q, k = p % L, p // L
     if L == 8:
         return 2 * q + (k & 1) + 16 * (k >> 1)

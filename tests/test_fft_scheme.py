@@ -198,6 +198,28 @@ def test_fft256_split_plane_scheme_matches_phaser_convention():
     numpy.testing.assert_allclose(_fft2_256(a, True), onum.ifft2(a), atol=1e-10)
 
 
+def test_split256_group_passes_compose_a_slice_step():
+    """The 256^2 group kernels (phx_split256.h) run a forward slice step as three passes
+    over the plane: a row pass (transmission, forward row transforms), a column pass
+    (forward column transforms, the propagator in natural order, inverse column
+    transforms) and a row pass (inverse row transforms, / n^2).  Composed from the line
+    model, this is psi_{i+1} = ifft2(fft2(psi O) p); the backward step's column pass with
+    conj(p) is its adjoint."""
+    n = 256
+    rng = numpy.random.default_rng(7)
+    psi, obj, prop = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) for _ in range(3))
+
+    def cols(a, inverse=False):
+        return numpy.stack([_line256(a[:, x], inverse) for x in range(n)]).T
+
+    def rows(a, inverse=False):
+        return numpy.stack([_line256(a[y], inverse) for y in range(n)])
+    out = rows(cols(cols(rows(psi * obj)) * prop, True), True) / (n * n)
+    numpy.testing.assert_allclose(out, numpy.fft.ifft2(numpy.fft.fft2(psi * obj) * prop), atol=1e-9)
+    back = rows(cols(cols(rows(out)) * prop.conj(), True), True) / (n * n)
+    numpy.testing.assert_allclose(numpy.vdot(back, psi * obj), numpy.vdot(out, out), rtol=1e-9)
+
+
 def _bfly32(u, v, k):
     """phx_fft.h bfly32 in float32: a = u + W v, b = u - W v, W = W_32^k, with the
     rotation in tangent form (g the larger of |Re W|, |Im W|, r the other over g)."""
