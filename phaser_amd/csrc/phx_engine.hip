@@ -2140,6 +2140,7 @@ int phx_epie_backward(const phx_group_desc* d, int slice, int update_object, int
     return PHX_EINVAL;
   hipLaunchKernelGGL(epie_maxima_k, dim3((unsigned)d->bmax), dim3(256), 0, stream, *d, slice);
   PHX_CHECK_LAUNCH();
+  if (d->n == S256) return s256_epie_backward(*d, slice, update_probe, stream);
 #define L(N, T)                                                                                           \
   hipLaunchKernelGGL((epie_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, update_object, \
                      update_probe)
@@ -2178,6 +2179,7 @@ int phx_epie_probe_apply(const phx_group_desc* d, hipStream_t stream) {
   // slice 0, summed in position order
   r = phx_probe_spectrum_reduce(d, stream);
   if (r) return r;
+  if (d->n == S256) return s256_epie_probe_apply(*d, stream);
 #define L(N, T) hipLaunchKernelGGL((epie_probe_apply_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -2209,6 +2211,7 @@ int phx_farfield_gradient(const phx_group_desc* d, int poisson, hipStream_t stre
   int r = check_desc(d);
   if (r) return r;
   if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a) return PHX_EINVAL;
+  if (d->n == S256) return s256_farfield_gradient(*d, poisson, stream);
 #define L(N, T) hipLaunchKernelGGL((farfield_fwd_k<N, T, false>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -2234,6 +2237,8 @@ int phx_gradient_backward(const phx_group_desc* d, int slice, int want_object, i
   if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (want_object && !d->grad_obj) ||
       ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)) || (want_probe && !d->farfield))
     return PHX_EINVAL;
+  // 256 x 256: the object gradient is only gathered (phx_gradient_object_gather), not scattered
+  if (d->n == S256) return want_object ? PHX_EUNSUPPORTED : s256_gradient_backward(*d, slice, want_probe, want_positions, stream);
 #define L(N, T)                                                                                              \
   hipLaunchKernelGGL((gradient_backward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, slice, want_object, \
                      want_probe, want_positions)
@@ -2251,6 +2256,7 @@ int phx_gradient_probe_finish(const phx_group_desc* d, hipStream_t stream) {
   // position order
   r = phx_probe_spectrum_reduce(d, stream);
   if (r) return r;
+  if (d->n == S256) return s256_gradient_probe_finish(*d, stream);
 #define L(N, T) \
   hipLaunchKernelGGL((gradient_probe_finish_k<N, T>), dim3((unsigned)d->n_modes), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);

@@ -110,6 +110,9 @@ enum S256Op {
   R_BACK,            // chi_i: step-size sums, dP = chi conj(O_i) -> forward rows -> chi_{i-1} or farfield
   R_BACK_FINISH,     // chi_{i-1} semi -> inverse rows / n^2 -> chi_{i-1}
   R_PROBE_UPDATE,    // d_probe semi -> inverse rows, P += update; P -> forward rows -> d_probe (per mode)
+  R_EPIE_BACK,       // ePIE: chi_i conj(O_i) / max|O|^2 -> forward rows -> chi_{i-1} or farfield
+  R_EPIE_PROBE,      // ePIE: d_probe semi -> inverse rows, P += beta_P v / (n^2 B); d_probe = 0 (per mode)
+  R_GRAD_PROBE,      // gradient engine: d_probe semi -> inverse rows / n^2 -> grad_probe; d_probe = 0 (per mode)
   // column passes
   C_PROBE_FWD,       // probe_k[m] forward columns in place                          (per mode)
   C_SHIFT,           // probe_k[m] filt_b -> inverse columns -> psi_0
@@ -119,6 +122,9 @@ enum S256Op {
   C_BACK_PROBE,      // forward columns of dP, * conj(filt_b) -> farfield (per-position probe spectrum)
   C_PROBE_INV,       // d_probe[m] inverse columns in place                          (per mode)
   C_PROBE_SPEC,      // d_probe[m] forward columns -> probe_k[m]; d_probe = 0        (per mode)
+  C_GRAD_AMPLITUDE,  // gradient engine: amplitude-loss gradient of farfield -> inverse columns -> chi_a, loss
+  C_GRAD_POISSON,    // the same for the Poisson loss
+  C_GRAD_PROBE,      // gradient engine, slice 0: forward columns -> probe-gradient spectrum, position gradients
 };
 
 // Row pass.  Grid (S256_ROW_TILES * M, bmax), or (S256_ROW_TILES * M, 1) for the
@@ -128,7 +134,7 @@ template <int OP>
 __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int flags) {
   __shared__ S256Lds<false> s;
   const int tile = blockIdx.x % S256_ROW_TILES, m = blockIdx.x / S256_ROW_TILES, b = blockIdx.y;
-  constexpr bool PER_MODE = OP == R_PROBE_FWD || OP == R_PROBE_UPDATE;
+  constexpr bool PER_MODE = OP == R_PROBE_FWD || OP == R_PROBE_UPDATE || OP == R_EPIE_PROBE || OP == R_GRAD_PROBE;
   if (!PER_MODE && !slot_active(d, b)) return;
   const S256Pass<false> g(tile);
   s256_load_twiddles<false>(s, d);
@@ -191,8 +197,9 @@ __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int 
   } else if constexpr (OP == R_BACK) {
     // LSQ-ML backward step at slice i (solvers.py:242-271), its first pass: the step-size
     // sums of the (pre-update) cutouts and dP = chi conj(O_i), transformed along rows into
-    // chi_{i-1}'s buffer (i > 0) or the farfield slot (i == 0 with a probe update)
-    const bool upd_obj = flags & 1, probe = flags & 2;
+    // chi_{i-1}'s buffer (i > 0) or the farfield slot (i == 0 with a probe update, or with
+    // flags bit 2: the gradient engine's adjoint, which takes no step-size sums)
+    const bool upd_obj = flags & 1, probe = flags & 2, spectrum = flags & 4;
     const Cut c = cut_of(d, d.group_pos[b]);
     const float2* chi = chi_in_buf(d, i) + plane;
     const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
@@ -231,7 +238,7 @@ __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int 
         atomic_add_f64(&d.red[L.denP], tb);
       }
     }
-    if (i > 0 || probe) {
+    if (i > 0 || probe || spectrum) {
       float2* out = i > 0 ? chi_out_buf(d, i) + plane : (float2*)d.farfield + plane;
       s256_line<false, false>(x, s, g.l, g.t);
 #pragma unroll
@@ -275,6 +282,51 @@ __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int 
     s256_line<false, false>(x, s, g.l, g.t);
 #pragma unroll
     for (int a = 0; a < 16; ++a) dPk[g.e(a)] = x[a];
+  } else if constexpr (OP == R_EPIE_BACK) {
+    // ePIE backward step at slice i (solvers.py:459-485), its first pass: the probe-side
+    // update chi conj(O_i) / max|O_i|^2 of the (pre-update) cutout, transformed along rows
+    // into chi_{i-1}'s buffer (i > 0) or the farfield slot (i == 0, probe update)
+    const Cut c = cut_of(d, d.group_pos[b]);
+    const float2* chi = chi_in_buf(d, i) + plane;
+    const float2* obj_i = (const float2*)d.object + i * HW;
+    const float inv_maxo = 1.0f / d.aux[b];
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = cscale(cmulc(chi[g.e(a)], obj_at(obj_i, c, g.y(a), g.x(a))), inv_maxo);
+    float2* out = i > 0 ? chi_out_buf(d, i) + plane : (float2*)d.farfield + plane;
+    s256_line<false, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) out[g.e(a)] = x[a];
+  } else if constexpr (OP == R_EPIE_PROBE) {
+    // ePIE probe step (solvers.py:481-484): P += beta_P ifft2(mean_b spectra)
+    const float sc = d.params[P_BETA_PROBE] * S256_INV_NN / (float)(*d.group_size);
+    float2* dPk = (float2*)d.d_probe + (size_t)m * S256_NN;
+    float2* P = (float2*)d.probe + (size_t)m * S256_NN;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      x[a] = dPk[g.e(a)];
+      dPk[g.e(a)] = make_float2(0.f, 0.f);  // d_probe is kept zero
+    }
+    s256_line<false, true>(x, s, g.l, g.t);
+    bool bad = false;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const float2 o = cadd(P[g.e(a)], cscale(x[a], sc));
+      P[g.e(a)] = o;
+      bad |= !(isfinite(o.x) && isfinite(o.y));
+    }
+    if (bad) atomicMin(d.finite_flag, d.group_index);
+  } else if constexpr (OP == R_GRAD_PROBE) {
+    // grad_probe[m] = ifft2(sum_b spectra) (the probe gradient), d_probe kept zero
+    float2* dPk = (float2*)d.d_probe + (size_t)m * S256_NN;
+    float2* gP = (float2*)d.grad_probe + (size_t)m * S256_NN;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      x[a] = dPk[g.e(a)];
+      dPk[g.e(a)] = make_float2(0.f, 0.f);
+    }
+    s256_line<false, true>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) gP[g.e(a)] = cscale(x[a], S256_INV_NN);
   }
 }
 
@@ -366,6 +418,62 @@ __global__ void __launch_bounds__(512) s256_cols_k(phx_group_desc d, int i, int 
     s256_line<true, true>(x, s, g.l, g.t);
 #pragma unroll
     for (int a = 0; a < 16; ++a) chi[g.e(a)] = x[a];
+  } else if constexpr (OP == C_GRAD_AMPLITUDE || OP == C_GRAD_POISSON) {
+    // gradient of the amplitude (noise_models.py:29-45) or Poisson (:86-102) loss w.r.t.
+    // the exit wave (torch's convention, times params[P_LOSS_SCALE]), then its inverse
+    // column transform into chi_a; the loss of the mode-0 planes goes to red.loss (FP64)
+    constexpr int FM = OP == C_GRAD_AMPLITUDE ? FM_GRAD_AMPLITUDE : FM_GRAD_POISSON;
+    const float2* F = (const float2*)d.farfield + plane;
+    const float* D = d.patterns_s + (size_t)d.group_pos[b] * S256_NN;
+    const float off = d.params[P_OFFSET], eps = d.params[P_EPS], var = d.params[P_VAR];
+    const float gscale = d.params[P_LOSS_SCALE];
+    float2* chi = (float2*)d.chi_a + plane;
+    float e2 = 0.f;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int e = g.e(a);
+      float t;
+      const float u = farfield_factor<FM>(model_intensity(d, b, e), D[e], d.mask_s[e], off, eps, var, gscale, t);
+      e2 += t;
+      x[a] = cscale(F[e], u);
+    }
+    if (m == 0) add_group_loss<FM>(d, e2, s.red);
+    s256_line<true, true>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) chi[g.e(a)] = x[a];
+  } else if constexpr (OP == C_GRAD_PROBE) {
+    // slice 0 of the adjoint: the probe-gradient spectrum fft2(g_psi0) conj(filt_b) into the
+    // farfield slot (flags bit 0; summed over positions in order by the probe finish) and
+    // the position gradients Re sum conj(fft2 g_psi0) fft2(P) filt (-2 pi i k) (bit 1)
+    const bool want_probe = flags & 1, want_pos = flags & 2;
+    const int p = d.group_pos[b];
+    const float sy = d.shift[2 * p], sx = d.shift[2 * p + 1];
+    const float2* Pk = (const float2*)d.probe_k + (size_t)m * S256_NN;
+    float2* Fb = (float2*)d.farfield + plane;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = Fb[g.e(a)];
+    s256_line<true, false>(x, s, g.l, g.t);
+    float gy = 0.f, gx = 0.f;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int e = g.e(a);
+      const float ky = d.ky_s[g.y(a)], kx = d.kx_s[g.x(a)];
+      const float2 f = shift_filter(sy, sx, ky, kx);
+      if (want_probe) Fb[e] = cmulc(x[a], f);
+      if (want_pos) {
+        const float2 t = cmulc(cmul(Pk[e], f), x[a]);  // conj(v) fft2(P) filt
+        gy += TWO_PI * ky * t.y;
+        gx += TWO_PI * kx * t.y;
+      }
+    }
+    if (want_pos) {
+      const double ty = s256_block_sum<true>(gy, s), tx = s256_block_sum<true>(gx, s);
+      if (threadIdx.x == 0) {
+        const RedLayout L = red_layout(d.n_slices, d.bmax, M);
+        atomic_add_f64(&d.red[L.pos_num + 2 * b], ty * (double)S256_INV_NN);
+        atomic_add_f64(&d.red[L.pos_num + 2 * b + 1], tx * (double)S256_INV_NN);
+      }
+    }
   } else if constexpr (OP == C_BACK_PROBE) {
     // per-position probe-update spectrum fft2(dP) conj(filt_b) (solvers.py:262-270), summed
     // over positions in order by probe_spectrum_reduce_k
@@ -464,6 +572,67 @@ inline int s256_lsqml_backward(const phx_group_desc& d, int i, int upd_obj, int 
   } else if (probe) {
     S256_COLS(C_BACK_PROBE, false, 0, 0);
   }
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// ePIE backward step at slice i (after epie_maxima_k): back-propagation (i > 0) or the
+// per-position probe spectrum (i == 0 with a probe update); at slice 0 without one there
+// is nothing to do (the object update is gathered by epie_object_update_k).
+inline int s256_epie_backward(const phx_group_desc& d, int i, int upd_probe, hipStream_t stream) {
+  if (i == 0 && !upd_probe) return PHX_OK;
+  S256_ROWS(R_EPIE_BACK, false, i, 0);
+  if (i > 0) {
+    S256_COLS(C_PROPAGATE, false, i - 1, 1);
+    S256_ROWS(R_BACK_FINISH, false, i, 0);
+  } else {
+    S256_COLS(C_BACK_PROBE, false, 0, 0);
+  }
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// ePIE probe step after the ordered sum of the per-position spectra into d_probe
+inline int s256_epie_probe_apply(const phx_group_desc& d, hipStream_t stream) {
+  S256_COLS(C_PROBE_INV, true, 0, 0);
+  S256_ROWS(R_EPIE_PROBE, true, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// Gradient engine: exit spectra, then the loss gradient w.r.t. the exit wave into chi_a
+// and the group loss
+inline int s256_farfield_gradient(const phx_group_desc& d, int poisson, hipStream_t stream) {
+  S256_ROWS(R_EXIT, false, 0, 0);
+  S256_COLS(C_EXIT, false, 0, 0);
+  if (poisson) S256_COLS(C_GRAD_POISSON, false, 0, 0);
+  else S256_COLS(C_GRAD_AMPLITUDE, false, 0, 0);
+  S256_ROWS(R_CHI_FINISH, false, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// Gradient engine adjoint step at slice i (the object gradient is gathered afterwards from
+// the wave gradients the steps leave in chi_in_buf): back-propagation (i > 0), or at
+// slice 0 the probe-gradient spectra and position gradients.
+inline int s256_gradient_backward(const phx_group_desc& d, int i, int want_probe, int want_pos,
+                                  hipStream_t stream) {
+  if (i > 0) {
+    S256_ROWS(R_BACK, false, i, 0);
+    S256_COLS(C_PROPAGATE, false, i - 1, 1);
+    S256_ROWS(R_BACK_FINISH, false, i, 0);
+  } else if (want_probe || want_pos) {
+    S256_ROWS(R_BACK, false, 0, 4);
+    S256_COLS(C_GRAD_PROBE, false, 0, (want_probe ? 1 : 0) | (want_pos ? 2 : 0));
+  }
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// grad_probe = ifft2(d_probe) after the ordered sum of the per-position spectra
+inline int s256_gradient_probe_finish(const phx_group_desc& d, hipStream_t stream) {
+  S256_COLS(C_PROBE_INV, true, 0, 0);
+  S256_ROWS(R_GRAD_PROBE, true, 0, 0);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

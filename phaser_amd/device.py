@@ -24,8 +24,9 @@ from .optics import fft_lanes, scr_to_freq, twiddles
 INT32_MAX = 2**31 - 1
 # largest plane whose LSQ-ML probe step runs in strips (phx_lsqml_probe_apply_strips)
 PROBE_STRIPS_MAX_N = 128
-# plane sides with group kernels; 256 x 256 planes run the LSQ-ML step only, in split
-# passes (row and column launches through HBM), without position correction
+# plane sides with group kernels; 256 x 256 planes run the conventional engines' steps
+# (LSQ-ML, ePIE) in split passes (row and column launches through HBM), without position
+# correction
 PLANE_SIDES = (16, 32, 64, 128, 192, 256)
 SPLIT_N = 256
 # largest plane side the whole-sweep kernels are used for (larger: one launch per slice)

@@ -227,9 +227,9 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     pattern_mask = numpy.asarray(args['data'].pattern_mask)
     assert (hasattr(patterns, 'is_cuda') or patterns.dtype == numpy.float32) and pattern_mask.dtype == numpy.float32
 
-    if int(state.probe.data.shape[-1]) == SPLIT_N and (props.solver.name != 'lsqml' or props.update_positions):
-        raise NotImplementedError(f"{SPLIT_N} x {SPLIT_N} planes: the MI355X backend runs LSQ-ML without position "
-                                  "correction at this size")
+    if int(state.probe.data.shape[-1]) == SPLIT_N and props.update_positions:
+        raise NotImplementedError(f"{SPLIT_N} x {SPLIT_N} planes: the MI355X backend has no position-correction "
+                                  "kernels at this size")
     solver = SOLVER_CLASSES[props.solver.name](props, props.solver, use_graphs=use_graphs)
     groups = GroupManager(state.scan, props.grouping, props.compact, seed=seed)
     calc_error_mask = mask_fraction_of_groups(len(groups), props.calc_error_fraction)

@@ -172,10 +172,10 @@ class MultisliceModel:
         return self.ws.red[g, o:o + 2 * self.ws.B].view(self.ws.B, 2)
 
 
-def _check_plane_side(state):
-    if int(state.probe.data.shape[-1]) == SPLIT_N:
-        raise NotImplementedError(f"{SPLIT_N} x {SPLIT_N} planes: the MI355X gradient engine has no adjoint kernels "
-                                  "at this size (the conventional LSQ-ML engine runs it)")
+def _check_plane_side(state, recompute):
+    if int(state.probe.data.shape[-1]) == SPLIT_N and recompute:
+        raise NotImplementedError(f"{SPLIT_N} x {SPLIT_N} planes: the MI355X gradient engine stores every slice's "
+                                  "wave at this size (no recompute mode)")
 
 
 def make_model(state, patterns, pattern_mask, noise, *, groups, bwlim_frac=2 / 3, recompute=False):
@@ -183,7 +183,7 @@ def make_model(state, patterns, pattern_mask, noise, *, groups, bwlim_frac=2 / 3
     (list of index arrays into the scan shape).  Returns (model, device state)."""
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
-    _check_plane_side(state)
+    _check_plane_side(state, recompute)
     props = make_propagators(state.probe.sampling, state.wavelength, state.object.thicknesses, bwlim_frac)
     bmax = max(numpy.asarray(g).shape[-1] for g in groups)
     ws = GroupWorkspace(state, host_patterns(patterns), numpy.asarray(pattern_mask), props,
@@ -364,7 +364,7 @@ def run_engine(args, props, *, recompute=False):
     seed = args.get('seed')
     if not hasattr(state.object.data, 'is_cuda'):
         state = state.to_device()
-    _check_plane_side(state)
+    _check_plane_side(state, recompute)
     noise = props.noise_model
     all_vars, group_solvers, iter_solvers = process_solvers(props)
     if 'tilt' in all_vars:

@@ -283,17 +283,25 @@ def test_n256_lsqml_split_passes(slices, modes, graphs):
            strict=slices > 1)
 
 
+def test_n256_epie_split_passes():
+    """ePIE at 256^2 (split row and column launches, object update gathered per tile):
+    three slices, two modes, two iterations against the oracle."""
+    _, state, pats = _problem(n=256, n_modes=2, n_slices=3, scan_shape=(3, 3), scan_step_px=10.0,
+                              jitter_px=0.5, seed=15)
+    _check(*_run_all(state, pats, solver='epie', niter=2, grouping=9, beta=0.5))
+
+
 def test_n256_engines_without_split_kernels_raise():
-    """At 256^2 only the LSQ-ML step has split-pass kernels: ePIE, LSQ-ML with position
-    correction and the gradient engine refuse the plan before running anything."""
+    """At 256^2 the engines run in split passes, but LSQ-ML position correction and the
+    gradient engine's recompute mode have no kernels there: those plans are refused
+    before anything runs."""
     from phaser_amd.engines.conventional import run_engine
     from phaser_amd.engines import gradient as geng
     from phaser_amd.plan import ConventionalEnginePlan, GradientEnginePlan, MomentumPositionSolverPlan
     _, state, pats = _problem(n=256, n_modes=1, n_slices=1, scan_shape=(2, 2), scan_step_px=10.0, jitter_px=0.0,
                               seed=3)
     args = {'data': pats, 'dtype': numpy.float32, 'seed': 7, 'observer': None, 'recons_name': 't'}
-    plans = [ConventionalEnginePlan(noise_model='amplitude', solver='epie', niter=1, grouping=4),
-             ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=1, grouping=4,
+    plans = [ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=1, grouping=4,
                                     position_solver=MomentumPositionSolverPlan(step_size=1e-2),
                                     update_positions=True)]
     for plan in plans:
@@ -302,7 +310,7 @@ def test_n256_engines_without_split_kernels_raise():
     gplan = GradientEnginePlan(noise_model='amplitude', niter=1, grouping=4,
                                solvers={'object': {'type': 'sgd', 'learning_rate': 1.0}})
     with pytest.raises(NotImplementedError):
-        geng.run_engine(dict(args, state=state.copy()), gplan)
+        geng.run_engine(dict(args, state=state.copy()), gplan, recompute=True)
 
 
 def test_lsqml_anscombe_multislice():

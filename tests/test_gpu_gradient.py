@@ -180,6 +180,55 @@ def test_autograd_function_backward():
     assert rel_err(pos.grad.cpu().numpy(), 2 * o_pos) < 1e-3
 
 
+@pytest.mark.parametrize('noise', ['amplitude', 'poisson'])
+def test_adjoint_n256_split_passes_match_oracle_gradient(noise):
+    """256^2 planes: the adjoint in split row/column launches (the far-field gradient, the
+    back-propagation, the probe-gradient spectra and the position gradients at slice 0),
+    the gathered object gradient and the ordered probe-gradient sum, against the
+    oracle's loss and gradients."""
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import NOISE_MODELS
+    _cfg, state, pats = _problem(S=2, M=2, n=256, scan=(2, 2), noise=noise)
+    group = numpy.array(numpy.unravel_index(numpy.arange(4), (2, 2)))
+    model, _ = make_model(state, pats.patterns, pats.pattern_mask, NOISE_MODELS[noise](), groups=[group])
+    model.forward(0)
+    model.backward(0)
+    loss = float(model.loss_tensor(0).item())
+    o_loss, o_obj, o_probe, o_pos = _oracle_grad(state, pats, group, noise)
+    e = (abs(loss - o_loss) / abs(o_loss), rel_err(model.ws.grad_obj.cpu().numpy(), o_obj),
+         rel_err(model.ws.grad_probe.view(model.ws.probe.shape).cpu().numpy(), o_probe),
+         rel_err(model.position_grads(0)[:4].cpu().numpy(), o_pos))
+    print(f"n=256 {noise}: loss {e[0]:.2e} object {e[1]:.2e} probe {e[2]:.2e} positions {e[3]:.2e}", flush=True)
+    assert e[0] < 1e-5 and e[1] < 1e-4 and e[2] < 1e-4 and e[3] < 1e-3
+
+
+def test_gradient_engine_n256_matches_oracle():
+    """The gradient engine (momentum SGD on object and probe) on 256^2 planes, three
+    iterations against the oracle's engine."""
+    from phaser_amd.engines.gradient import run_engine
+    from phaser_amd.plan import GradientEnginePlan, SGDSolverPlan
+    _cfg, state, pats = _problem(S=2, M=2, n=256, scan=(3, 3), seed=24)
+    state.probe.data = (state.probe.data / numpy.float32(numpy.sqrt(1e5))).astype(numpy.complex64)
+    plan = GradientEnginePlan(noise_model='amplitude',
+                              solvers={('object', 'probe'): SGDSolverPlan(learning_rate=0.5, momentum=0.5)},
+                              niter=3, grouping=9)
+    g = run_engine({'data': pats, 'state': state.copy(), 'seed': 3, 'dtype': numpy.float32}, plan).to_numpy()
+    res = []
+    for dt in (numpy.float32, numpy.float64):
+        o = to_oracle_state(state, dtype=dt)
+        nz = {'name': 'amplitude', 'offset': 0.0, 'eps': 1e-3, 'gaussian_variance': 0.1}
+        og.run_engine(o, pats.patterns.astype(dt), pats.pattern_mask.astype(dt), noise=nz,
+                      solvers=[og.Solver('sgd', ['object', 'probe'], learning_rate=0.5, momentum=0.5, nesterov=True)],
+                      niter=3, grouping=9, seed=3, dtype=dt)
+        res.append(o)
+    o32, o64 = res
+    for name, f in (('object', lambda s: s.object.data), ('probe', lambda s: s.probe.data),
+                    ('loss', lambda s: s.progress.detector_errors)):
+        a, b, c = rel_err(f(g), f(o32)), rel_err(f(g), f(o64)), rel_err(f(o32), f(o64))
+        print(f"n=256 {name}: gpu-vs-fp32 {a:.2e} gpu-vs-fp64 {b:.2e} fp32-vs-fp64 {c:.2e}", flush=True)
+        assert b <= 3 * c + 1e-4
+
+
 @pytest.mark.parametrize('solver', ['sgd', 'adam', 'polyak_sgd'])
 def test_gradient_engine_matches_oracle(solver):
     import oracle.state as ost
