@@ -273,7 +273,10 @@ int phx_epie_probe_apply(const phx_group_desc* d, hipStream_t stream);
 int phx_dry_run_stats(const phx_group_desc* d, hipStream_t stream);
 
 /* Position gradient numerators/denominators (solvers.py:275-288) for direction dir
- * (0 = y, 1 = x), using the final chi (in chi_a when S is odd, chi_b when even). */
+ * (0 = y, 1 = x), using the final chi (in chi_a when S is odd, chi_b when even).  At
+ * n = 256 the farfield slots are its work plane: call it before phx_lsqml_backward of
+ * slice 0 (which writes the probe spectra there; the step reads only chi_0, psi_0 and
+ * the pre-update slice-0 cutout). */
 int phx_position_gradient(const phx_group_desc* d, int dir, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------------

@@ -2200,6 +2200,7 @@ int phx_position_gradient(const phx_group_desc* d, int dir, hipStream_t stream) 
   int r = check_desc(d);
   if (r) return r;
   if ((dir != 0 && dir != 1) || !d->chi_a || !d->chi_b) return PHX_EINVAL;
+  if (d->n == S256) return d->farfield ? s256_position_gradient(*d, dir, stream) : PHX_EINVAL;
 #define L(N, T) hipLaunchKernelGGL((position_gradient_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, dir)
   PHX_DISPATCH(d->n, L);
 #undef L

@@ -113,6 +113,8 @@ enum S256Op {
   R_EPIE_BACK,       // ePIE: chi_i conj(O_i) / max|O|^2 -> forward rows -> chi_{i-1} or farfield
   R_EPIE_PROBE,      // ePIE: d_probe semi -> inverse rows, P += beta_P v / (n^2 B); d_probe = 0 (per mode)
   R_GRAD_PROBE,      // gradient engine: d_probe semi -> inverse rows / n^2 -> grad_probe; d_probe = 0 (per mode)
+  R_POS_START,       // LSQ-ML position step: psi_0 -> forward rows -> farfield (work plane)
+  R_POS_FINISH,      // inverse rows of d psi_0 / d pos, times O_0, dotted with chi_0 -> position sums
   // column passes
   C_PROBE_FWD,       // probe_k[m] forward columns in place                          (per mode)
   C_SHIFT,           // probe_k[m] filt_b -> inverse columns -> psi_0
@@ -125,6 +127,7 @@ enum S256Op {
   C_GRAD_AMPLITUDE,  // gradient engine: amplitude-loss gradient of farfield -> inverse columns -> chi_a, loss
   C_GRAD_POISSON,    // the same for the Poisson loss
   C_GRAD_PROBE,      // gradient engine, slice 0: forward columns -> probe-gradient spectrum, position gradients
+  C_POS,             // LSQ-ML position step: forward columns, * (-2 pi i k), inverse columns, in place
 };
 
 // Row pass.  Grid (S256_ROW_TILES * M, bmax), or (S256_ROW_TILES * M, 1) for the
@@ -315,6 +318,41 @@ __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int 
       bad |= !(isfinite(o.x) && isfinite(o.y));
     }
     if (bad) atomicMin(d.finite_flag, d.group_index);
+  } else if constexpr (OP == R_POS_START) {
+    const float2* psi0 = (const float2*)d.psi + psi_off(d, 0, b, m);
+    float2* w = (float2*)d.farfield + plane;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = psi0[g.e(a)];
+    s256_line<false, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) w[g.e(a)] = x[a];
+  } else if constexpr (OP == R_POS_FINISH) {
+    // LSQ-ML position step (solvers.py:275-288) in direction flags (0: y, 1: x):
+    // prod = ifft2(fft2(psi_0) (-2 pi i k)) / sum_m |psi_0|^2 * O_0 (pre-update cutout),
+    // num[b] += Re sum conj(chi_0) prod, den += sum |prod|^2 (FP64)
+    const int dir = flags & 1;
+    const RedLayout L = red_layout(d.n_slices, bmax, M);
+    const Cut c = cut_of(d, d.group_pos[b]);
+    const float2* w = (const float2*)d.farfield + plane;
+    const float2* chi = chi_in_buf(d, 0) + plane;
+    const float2* obj0 = (const float2*)d.object;
+    const float scale = (float)(1.0 / ((double)S256_NN * d.red[L.norm + b]));
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = w[g.e(a)];
+    s256_line<false, true>(x, s, g.l, g.t);
+    float num = 0.f, den = 0.f;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const float2 prod = cmul(cscale(x[a], scale), obj_at(obj0, c, g.y(a), g.x(a)));
+      const float2 cv = chi[g.e(a)];
+      num += cv.x * prod.x + cv.y * prod.y;
+      den += abs2f(prod);
+    }
+    const double ta = s256_block_sum<false>(num, s), tb = s256_block_sum<false>(den, s);
+    if (threadIdx.x == 0) {
+      atomic_add_f64(&d.red[L.pos_num + 2 * b + dir], ta);
+      atomic_add_f64(&d.red[L.pos_den + dir], tb);
+    }
   } else if constexpr (OP == R_GRAD_PROBE) {
     // grad_probe[m] = ifft2(sum_b spectra) (the probe gradient), d_probe kept zero
     float2* dPk = (float2*)d.d_probe + (size_t)m * S256_NN;
@@ -474,6 +512,20 @@ __global__ void __launch_bounds__(512) s256_cols_k(phx_group_desc d, int i, int 
         atomic_add_f64(&d.red[L.pos_num + 2 * b + 1], tx * (double)S256_INV_NN);
       }
     }
+  } else if constexpr (OP == C_POS) {
+    const int dir = flags & 1;
+    float2* w = (float2*)d.farfield + plane;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = w[g.e(a)];
+    s256_line<true, false>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const float wk = TWO_PI * (dir == 0 ? d.ky_s[g.y(a)] : d.kx_s[g.x(a)]);
+      x[a] = make_float2(wk * x[a].y, -wk * x[a].x);  // v * (-2 pi i k)
+    }
+    s256_line<true, true>(x, s, g.l, g.t);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) w[g.e(a)] = x[a];
   } else if constexpr (OP == C_BACK_PROBE) {
     // per-position probe-update spectrum fft2(dP) conj(filt_b) (solvers.py:262-270), summed
     // over positions in order by probe_spectrum_reduce_k
@@ -633,6 +685,17 @@ inline int s256_gradient_backward(const phx_group_desc& d, int i, int want_probe
 inline int s256_gradient_probe_finish(const phx_group_desc& d, hipStream_t stream) {
   S256_COLS(C_PROBE_INV, true, 0, 0);
   S256_ROWS(R_GRAD_PROBE, true, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// LSQ-ML position step in direction dir.  Its work plane is the farfield slot, so at this
+// size it runs before the slice-0 backward step writes the probe spectra there (it reads
+// only chi_0, psi_0 and the pre-update slice-0 cutout).
+inline int s256_position_gradient(const phx_group_desc& d, int dir, hipStream_t stream) {
+  S256_ROWS(R_POS_START, false, 0, 0);
+  S256_COLS(C_POS, false, 0, dir);
+  S256_ROWS(R_POS_FINISH, false, 0, dir);
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }

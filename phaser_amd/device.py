@@ -24,9 +24,8 @@ from .optics import fft_lanes, scr_to_freq, twiddles
 INT32_MAX = 2**31 - 1
 # largest plane whose LSQ-ML probe step runs in strips (phx_lsqml_probe_apply_strips)
 PROBE_STRIPS_MAX_N = 128
-# plane sides with group kernels; 256 x 256 planes run the conventional engines' steps
-# (LSQ-ML, ePIE) in split passes (row and column launches through HBM), without position
-# correction
+# plane sides with group kernels; 256 x 256 planes run every engine step in split passes
+# (row and column launches through HBM; the gradient engine without its recompute mode)
 PLANE_SIDES = (16, 32, 64, 128, 192, 256)
 SPLIT_N = 256
 # largest plane side the whole-sweep kernels are used for (larger: one launch per slice)
@@ -450,9 +449,15 @@ class GroupWorkspace:
         if update_probe:
             c("phx_group_obj_mag", d, st)
         deferred = self.chi_slices is not None or self.S == 1
+        split = self.n == SPLIT_N
         for i in range(self.S - 1, -1, -1):
+            if i == 0 and update_position and split:
+                # split planes: the position step's work plane is the farfield slot, which
+                # slice 0's step then fills with the probe spectra (the position step reads
+                # only chi_0, psi_0 and the pre-update slice-0 cutout)
+                self._launch_position(d, st)
             c("phx_lsqml_backward", d, i, int(update_object), int(update_probe), st)
-            if i == 0 and update_position:
+            if i == 0 and update_position and not split:
                 # the position step uses the group's pre-update slice-0 cutout and the
                 # final chi (solvers.py:275-288), so it runs before slice 0 is updated
                 self._launch_position(d, st)

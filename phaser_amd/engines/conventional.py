@@ -18,7 +18,7 @@ import logging
 import numpy
 
 from ..constraints import make_constraints
-from ..device import SPLIT_N, GroupWorkspace, host_patterns
+from ..device import GroupWorkspace, host_patterns
 from ..groups import GroupManager, mask_fraction_of_groups
 from ..optics import make_propagators
 from ..plan import process_flag, process_schedule, shuffle_groups_flag
@@ -227,9 +227,6 @@ def run_engine(args, props, *, use_graphs=True, keep_chi=None, fused=True):
     pattern_mask = numpy.asarray(args['data'].pattern_mask)
     assert (hasattr(patterns, 'is_cuda') or patterns.dtype == numpy.float32) and pattern_mask.dtype == numpy.float32
 
-    if int(state.probe.data.shape[-1]) == SPLIT_N and props.update_positions:
-        raise NotImplementedError(f"{SPLIT_N} x {SPLIT_N} planes: the MI355X backend has no position-correction "
-                                  "kernels at this size")
     solver = SOLVER_CLASSES[props.solver.name](props, props.solver, use_graphs=use_graphs)
     groups = GroupManager(state.scan, props.grouping, props.compact, seed=seed)
     calc_error_mask = mask_fraction_of_groups(len(groups), props.calc_error_fraction)

@@ -291,22 +291,32 @@ def test_n256_epie_split_passes():
     _check(*_run_all(state, pats, solver='epie', niter=2, grouping=9, beta=0.5))
 
 
-def test_n256_engines_without_split_kernels_raise():
-    """At 256^2 the engines run in split passes, but LSQ-ML position correction and the
-    gradient engine's recompute mode have no kernels there: those plans are refused
-    before anything runs."""
-    from phaser_amd.engines.conventional import run_engine
+def test_n256_lsqml_position_correction():
+    """LSQ-ML with momentum position correction at 256^2 (the position step runs in split
+    passes through the farfield slots, before slice 0's probe spectra go there): scan,
+    object and probe against the oracle over two iterations."""
+    _, state, pats = _problem(n=256, n_modes=2, n_slices=2, scan_shape=(3, 3), scan_step_px=10.0,
+                              jitter_px=0.5, seed=16)
+    pos = {'step_size': 1e-2, 'max_step_size': 0.1, 'momentum': 0.9}
+    g, o32, o64 = _run_all(state, pats, solver='lsqml', niter=2, grouping=9, beta=0.5, position=pos)
+    _check(g, o32, o64)
+    # the scan update on the FP64-relative bar, as at the smaller sizes
+    d_g, d_o, d_64 = g.scan - state.scan, o32.scan - state.scan, o64.scan - state.scan
+    e_g64, e_o64 = rel_err(d_g, d_64), rel_err(d_o, d_64)
+    print(f"scan update: gpu-vs-fp32 {rel_err(d_g, d_o):.2e} gpu-vs-fp64 {e_g64:.2e} fp32-vs-fp64 {e_o64:.2e}",
+          flush=True)
+    assert numpy.abs(d_o).max() > 0
+    assert e_g64 <= 3 * e_o64 + 1e-5
+
+
+def test_n256_gradient_recompute_mode_raises():
+    """At 256^2 the gradient engine keeps every slice's wave; its recompute mode has no
+    split-pass kernels and is refused before anything runs."""
     from phaser_amd.engines import gradient as geng
-    from phaser_amd.plan import ConventionalEnginePlan, GradientEnginePlan, MomentumPositionSolverPlan
+    from phaser_amd.plan import GradientEnginePlan
     _, state, pats = _problem(n=256, n_modes=1, n_slices=1, scan_shape=(2, 2), scan_step_px=10.0, jitter_px=0.0,
                               seed=3)
     args = {'data': pats, 'dtype': numpy.float32, 'seed': 7, 'observer': None, 'recons_name': 't'}
-    plans = [ConventionalEnginePlan(noise_model='amplitude', solver='lsqml', niter=1, grouping=4,
-                                    position_solver=MomentumPositionSolverPlan(step_size=1e-2),
-                                    update_positions=True)]
-    for plan in plans:
-        with pytest.raises(NotImplementedError):
-            run_engine(dict(args, state=state.copy()), plan)
     gplan = GradientEnginePlan(noise_model='amplitude', niter=1, grouping=4,
                                solvers={'object': {'type': 'sgd', 'learning_rate': 1.0}})
     with pytest.raises(NotImplementedError):
