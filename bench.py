@@ -391,11 +391,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-graphs", action="store_true")
-    ap.add_argument("--workload", choices=("128x4-s8", "192x8-s24", "128x6-s16-grad"), default="128x4-s8",
+    ap.add_argument("--workload", choices=("128x4-s8", "192x8-s24", "128x6-s16-grad", "256x4-s8"),
+                    default="128x4-s8",
                     help="128x6-s16-grad: the gradient engine at 16 slices of 1030^2, 128x128 x 6 modes, 16384 "
                          "positions at 7 px, position correction on (its own JSON line); "
                          "192x8-s24: the same LSQ-ML measurement at 24 slices, 192x192 x 8 modes, 36864 "
-                         "positions (no CPU baseline; its time-to-NRMSE leg with --ttn)")
+                         "positions (no CPU baseline; its time-to-NRMSE leg with --ttn); "
+                         "256x4-s8: the same at 8 slices, 256x256 x 4 modes (split row/column passes), "
+                         "1024 positions")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-groups", type=int, default=8)
     ap.add_argument("--no-learned", action="store_true",
@@ -449,11 +452,12 @@ def main():
     from phaser_amd.groups import GroupManager
     from phaser_amd.optics import make_propagators
     from phaser_amd.plan import ConventionalEnginePlan
-    from phaser_amd.synthetic import CONFIG_2, LSQML_192X8_S24, make_dataset
+    from phaser_amd.synthetic import CONFIG_2, LSQML_192X8_S24, LSQML_256X4_S8, make_dataset
 
     big = args.workload == "192x8-s24"
-    cfg = LSQML_192X8_S24 if big else CONFIG_2
-    if big:
+    split = args.workload == "256x4-s8"
+    cfg = LSQML_192X8_S24 if big else (LSQML_256X4_S8 if split else CONFIG_2)
+    if big or split:
         args.no_cpu = args.no_learned = True
     log("generating synthetic data (GPU forward model + Poisson noise)")
     t0 = time.perf_counter()
@@ -525,7 +529,7 @@ def main():
     traffic = None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                          "lsqml_multislice_512_b64_traffic.json")
-    if os.path.exists(tpath) and not big:
+    if os.path.exists(tpath) and not (big or split):
         with open(tpath) as f:
             traffic = json.load(f)['kernels'].get(dom, {}).get('traffic_bytes')
     whole = per_pi * value / 1e9

@@ -114,43 +114,41 @@ __device__ __forceinline__ float model_intensity(const phx_group_desc& d, int b,
   return s;
 }
 
+// model_intensity for up to four modes, as a per-point functor whose four loads (modes
+// past M re-read mode 0 and are not added) go out with the caller's other loads of the
+// point: no separate round trip, and the adds keep mode order.
+struct ModeSum4 {
+  const float *i0, *i1, *i2, *i3;
+  bool h1, h2, h3;
+  __device__ __forceinline__ ModeSum4(const phx_group_desc& d, int b) {
+    const size_t NN = (size_t)d.n * d.n;
+    const int M = d.n_modes;
+    i0 = d.intensity + (size_t)b * M * NN;
+    i1 = i0 + (M > 1 ? NN : 0);
+    i2 = i0 + (M > 2 ? 2 * NN : 0);
+    i3 = i0 + (M > 3 ? 3 * NN : 0);
+    h1 = M > 1;
+    h2 = M > 2;
+    h3 = M > 3;
+  }
+  __device__ __forceinline__ float operator()(int e) const {
+    const float a0 = i0[e], a1 = i1[e], a2 = i2[e], a3 = i3[e];
+    float s = a0;
+    s = h1 ? s + a1 : s;
+    s = h2 ? s + a2 : s;
+    s = h3 ? s + a3 : s;
+    return s;
+  }
+};
+
 // model_intensity at every column-phase point of this lane (register k: py = QL k + q,
-// px = rc).  Up to four modes' loads for all the points are issued before the first add
-// (one memory round trip, not one per mode); the adds keep mode order.
+// px = rc), for more than four modes: each mode's loads for all the points together.
 template <int N>
 __device__ __forceinline__ void model_intensity_cols(const phx_group_desc& d, int b, int q, int rc,
                                                      float (&Iv)[Plane<N>::E]) {
   constexpr int NN = N * N, E = Plane<N>::E, QL = Plane<N>::QL;
   const int M = d.n_modes;
   const float* I = d.intensity + (size_t)b * M * NN + q * N + rc;
-  if (M == 1) {
-#pragma unroll
-    for (int k = 0; k < E; ++k) Iv[k] = I[k * QL * N];
-    return;
-  }
-  if (M <= 4) {
-    // modes 1..3 (a mode past M re-reads mode 0 and is not added)
-    const float* I1 = I + (M > 1 ? NN : 0);
-    const float* I2 = I + (M > 2 ? 2 * NN : 0);
-    const float* I3 = I + (M > 3 ? 3 * NN : 0);
-    float t1[E], t2[E], t3[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      Iv[k] = I[k * QL * N];
-      t1[k] = I1[k * QL * N];
-      t2[k] = I2[k * QL * N];
-      t3[k] = I3[k * QL * N];
-    }
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      float s = Iv[k];
-      if (M > 1) s += t1[k];
-      if (M > 2) s += t2[k];
-      if (M > 3) s += t3[k];
-      Iv[k] = s;
-    }
-    return;
-  }
 #pragma unroll
   for (int k = 0; k < E; ++k) Iv[k] = I[k * QL * N];
   for (int mm = 1; mm < M; ++mm) {
@@ -695,19 +693,28 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
   const float* mask = d.mask_s;
   const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
   float e2 = 0.f;
-  float Ik[E];
-  model_intensity_cols<N>(d, b, pl.q, pl.rc, Ik);
-  int kk = 0;  // register index of the point cols_apply passes (its loop is unrolled)
-  pl.cols_apply(x, [&](int py, int px, float2) {
-    const int e = py * N + px;
-    const float Iv = Ik[kk++], Dv = D[e], mk = mask[e];
-    const float2 Fv = F[e];
-    const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
-    const float df = Iv - Dv;
-    const float t = df * df;  // unmasked (solvers.py:237), NaN-skipping (nansum)
-    e2 += isnan(t) ? 0.f : t;
-    return cscale(Fv, (sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f) * mk);
-  }, [&] { __asm__ volatile("" : "+v"(e2)); });
+  // isum(e): the model intensity of point e (mode terms added in mode order)
+  auto farfield_phase = [&](auto isum) __attribute__((always_inline)) {
+    pl.cols_apply(x, [&](int py, int px, float2) {
+      const int e = py * N + px;
+      const float Iv = isum(e), Dv = D[e], mk = mask[e];
+      const float2 Fv = F[e];
+      const float Dp = (Dv > 0.f || isnan(Dv)) ? Dv : 0.f;  // numpy.maximum keeps NaN
+      const float df = Iv - Dv;
+      const float t = df * df;  // unmasked (solvers.py:237), NaN-skipping (nansum)
+      e2 += isnan(t) ? 0.f : t;
+      return cscale(Fv, (sqrtf(Dp + off) / (sqrtf(Iv + off) + eps) - 1.0f) * mk);
+    }, [&] { __asm__ volatile("" : "+v"(e2)); });
+  };
+  if (M <= 4) {
+    const ModeSum4 ms(d, b);
+    farfield_phase([&](int e) { return ms(e); });
+  } else {
+    float Ik[E];
+    model_intensity_cols<N>(d, b, pl.q, pl.rc, Ik);
+    int kk = 0;  // register index of the point cols_apply passes (its loop is unrolled)
+    farfield_phase([&](int) { return Ik[kk++]; });
+  }
   if (calc_error && m == 0) {
     const double tot = block_sum_f64(e2, red_lds_);
     if (threadIdx.x == 0) d.errors[b] = (float)sqrt(tot);
@@ -832,18 +839,27 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
     const float off = d.params[P_OFFSET], eps = d.params[P_EPS], var = d.params[P_VAR];
     const float gscale = d.params[P_LOSS_SCALE];
     float e2 = 0.f;
-    float Ik[E];
-    model_intensity_cols<N>(d, b, pl.q, pl.rc, Ik);
-    int kk = 0;  // register index of the point cols_apply passes (its loop is unrolled)
-    pl.cols_apply(x, [&](int py, int px, float2) {
-      const int e = py * N + px;
-      const float Iv = Ik[kk++], Dv = D[e], mk = mask[e];
-      const float2 Fv = F[e];
-      float t;
-      const float u = farfield_factor<FF>(Iv, Dv, mk, off, eps, var, gscale, t);
-      e2 += t;
-      return cscale(Fv, u);
-    }, [&] { __asm__ volatile("" : "+v"(e2)); });
+    // isum(e): the model intensity of point e (mode terms added in mode order)
+    auto farfield_phase = [&](auto isum) __attribute__((always_inline)) {
+      pl.cols_apply(x, [&](int py, int px, float2) {
+        const int e = py * N + px;
+        const float Iv = isum(e), Dv = D[e], mk = mask[e];
+        const float2 Fv = F[e];
+        float t;
+        const float u = farfield_factor<FF>(Iv, Dv, mk, off, eps, var, gscale, t);
+        e2 += t;
+        return cscale(Fv, u);
+      }, [&] { __asm__ volatile("" : "+v"(e2)); });
+    };
+    if (M <= 4) {
+      const ModeSum4 ms(d, b);
+      farfield_phase([&](int e) { return ms(e); });
+    } else {
+      float Ik[E];
+      model_intensity_cols<N>(d, b, pl.q, pl.rc, Ik);
+      int kk = 0;  // register index of the point cols_apply passes (its loop is unrolled)
+      farfield_phase([&](int) { return Ik[kk++]; });
+    }
     if (m == 0) add_group_loss<FF>(d, e2, red_lds_);
     pl.inverse_regs(x);
   }

@@ -44,6 +44,9 @@ CONFIG_2 = SyntheticConfig(n=128, n_modes=4, n_slices=8, scan_shape=(64, 64), sc
 # 24 slices, 192x192 probe x 8 modes, 192 x 192 positions at 7 px, Poisson at 2e4
 LSQML_192X8_S24 = SyntheticConfig(n=192, n_modes=8, n_slices=24, scan_shape=(192, 192), scan_step_px=7.0,
                                   jitter_px=0.0, dose=2e4)
+# 8 slices, 256x256 probe x 4 modes (the split-pass plane size), 32 x 32 positions at 12 px
+LSQML_256X4_S8 = SyntheticConfig(n=256, n_modes=4, n_slices=8, scan_shape=(32, 32), scan_step_px=12.0,
+                                 jitter_px=0.0)
 
 
 def raster_scan(cfg: SyntheticConfig):
