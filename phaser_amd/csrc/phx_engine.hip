@@ -21,7 +21,7 @@ namespace {
 constexpr float TWO_PI = 6.283185307179586f;
 
 struct RedLayout {
-  int num_obj, den_obj, numP, denP, norm, pos_num, pos_den, loss, total;
+  int num_obj, den_obj, numP, denP, norm, pos_num, pos_den, loss, err, total;
 };
 __host__ __device__ inline RedLayout red_layout(int S, int bmax, int M) {
   RedLayout r;
@@ -34,6 +34,7 @@ __host__ __device__ inline RedLayout red_layout(int S, int bmax, int M) {
   r.pos_num = o; o += 2 * bmax;
   r.pos_den = o; o += 2;
   r.loss = o;    o += 1;
+  r.err = o;     o += 8 * bmax;  // 256 x 256 planes: per-slot error partials of the 8 column tiles
   r.total = o;
   return r;
 }
@@ -1886,6 +1887,7 @@ int phx_group_forward(const phx_group_desc* d, hipStream_t stream) {
   if (r) return r;
   if (!d->probe_k || !d->farfield || !d->intensity || (d->n_slices > 1 && !d->props)) return PHX_EINVAL;
   if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // stores every slice
+  if (d->n == S256) return s256_group_forward(*d, stream);
 #define L(N, T) hipLaunchKernelGGL((group_forward_k<N, T>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d)
   PHX_DISPATCH(d->n, L);
 #undef L
@@ -1898,6 +1900,7 @@ int phx_farfield_update(const phx_group_desc* d, int calc_error, hipStream_t str
   if (r) return r;
   if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a || (calc_error && !d->errors))
     return PHX_EINVAL;
+  if (d->n == S256) return s256_farfield_update(*d, calc_error, stream);
 #define L(N, T) \
   hipLaunchKernelGGL((farfield_update_k<N, T, FM_MODULUS>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error)
   PHX_DISPATCH(d->n, L);
@@ -1914,6 +1917,7 @@ int phx_lsqml_backward_sweep(const phx_group_desc* d, int update_object, int upd
       (d->n_slices > 1 && (!d->props || !d->chi_slices)))
     return PHX_EINVAL;
   if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;
+  if (d->n == S256) return s256_lsqml_backward_sweep(*d, update_object, update_probe, calc_error, stream);
 #define K(N, T, UO, PR) \
   hipLaunchKernelGGL((lsqml_backward_fused_k<N, T, UO, PR>), PHX_PLANE_GRID(d), dim3(T), 0, stream, *d, calc_error)
 #define L(N, T)                       \
@@ -1938,6 +1942,7 @@ int phx_farfield_gradient_update(const phx_group_desc* d, int poisson, hipStream
   int r = check_desc(d);
   if (r) return r;
   if (!d->farfield || !d->intensity || !d->patterns_s || !d->mask_s || !d->chi_a) return PHX_EINVAL;
+  if (d->n == S256) return s256_farfield_gradient_update(*d, poisson, stream);
 #define L(N, T)                                                                                                  \
   do {                                                                                                           \
     if (poisson)                                                                                                 \

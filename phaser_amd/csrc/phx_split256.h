@@ -103,11 +103,12 @@ enum S256Op {
   // row passes
   R_PROBE_FWD = 0,   // probe[m] -> forward rows -> probe_k[m]                        (per mode)
   R_SHIFT_FINISH,    // psi_0 semi -> inverse rows / n^2 -> psi_0, norm; [* O_0 -> fwd rows -> psi_1]
-  R_SLICE_START,     // psi_i * O_i -> forward rows -> psi_{i+1}
+  R_SLICE_START,     // [psi_i semi -> inverse rows / n^2 -> psi_i;] psi_i * O_i -> forward rows -> psi_{i+1}
   R_WAVE_FINISH,     // psi_{i+1} semi -> inverse rows / n^2 -> psi_{i+1}
   R_EXIT,            // psi_{S-1} * O_{S-1} -> forward rows -> farfield
   R_CHI_FINISH,      // chi_a semi -> inverse rows / n^2 -> chi_a
-  R_BACK,            // chi_i: step-size sums, dP = chi conj(O_i) -> forward rows -> chi_{i-1} or farfield
+  R_BACK,            // [chi_i semi -> inverse rows / n^2 -> chi_i;] step-size sums, dP = chi conj(O_i)
+                     //   -> forward rows -> chi_{i-1} or farfield
   R_BACK_FINISH,     // chi_{i-1} semi -> inverse rows / n^2 -> chi_{i-1}
   R_PROBE_UPDATE,    // d_probe semi -> inverse rows, P += update; P -> forward rows -> d_probe (per mode)
   R_EPIE_BACK,       // ePIE: chi_i conj(O_i) / max|O|^2 -> forward rows -> chi_{i-1} or farfield
@@ -132,7 +133,10 @@ enum S256Op {
 
 // Row pass.  Grid (S256_ROW_TILES * M, bmax), or (S256_ROW_TILES * M, 1) for the
 // per-mode operations; 256 threads.  `i` is the slice; flags: bit 0 propagate
-// (R_SHIFT_FINISH) / object sums (R_BACK), bit 1 probe sums and spectrum (R_BACK).
+// (R_SHIFT_FINISH) / object sums (R_BACK), bit 1 probe sums and spectrum (R_BACK), bit 2
+// the slice-0 spectrum without sums (R_BACK), bit 3 finish first (R_SLICE_START, R_BACK:
+// the slice's wave or chi arrives from a column pass and its inverse row transform is
+// completed, stored, and used in the same pass).
 template <int OP>
 __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int flags) {
   __shared__ S256Lds<false> s;
@@ -187,13 +191,24 @@ __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int 
       }
     }
   } else if constexpr (OP == R_SLICE_START || OP == R_EXIT) {
+    // the last slice's product goes to the farfield slot (the exit wave's first pass)
     const int si = OP == R_EXIT ? d.n_slices - 1 : i;
     const Cut c = cut_of(d, d.group_pos[b]);
-    const float2* psi_i = (const float2*)d.psi + psi_off(d, si, b, m);
+    float2* psi_i = (float2*)d.psi + psi_off(d, si, b, m);
     const float2* obj_i = (const float2*)d.object + si * HW;
-    float2* out = OP == R_EXIT ? (float2*)d.farfield + plane : (float2*)d.psi + psi_off(d, si + 1, b, m);
+    float2* out = si == d.n_slices - 1 ? (float2*)d.farfield + plane : (float2*)d.psi + psi_off(d, si + 1, b, m);
 #pragma unroll
-    for (int a = 0; a < 16; ++a) x[a] = cmul(psi_i[g.e(a)], obj_at(obj_i, c, g.y(a), g.x(a)));
+    for (int a = 0; a < 16; ++a) x[a] = psi_i[g.e(a)];
+    if (flags & 8) {
+      s256_line<false, true>(x, s, g.l, g.t);
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        x[a] = cscale(x[a], S256_INV_NN);
+        psi_i[g.e(a)] = x[a];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = cmul(x[a], obj_at(obj_i, c, g.y(a), g.x(a)));
     s256_line<false, false>(x, s, g.l, g.t);
 #pragma unroll
     for (int a = 0; a < 16; ++a) out[g.e(a)] = x[a];
@@ -204,13 +219,23 @@ __global__ void __launch_bounds__(256) s256_rows_k(phx_group_desc d, int i, int 
     // flags bit 2: the gradient engine's adjoint, which takes no step-size sums)
     const bool upd_obj = flags & 1, probe = flags & 2, spectrum = flags & 4;
     const Cut c = cut_of(d, d.group_pos[b]);
-    const float2* chi = chi_in_buf(d, i) + plane;
+    float2* chi = chi_in_buf(d, i) + plane;
     const float2* psi_i = (const float2*)d.psi + psi_off(d, i, b, m);
     const float2* obj_i = (const float2*)d.object + i * HW;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) x[a] = chi[g.e(a)];
+    if (flags & 8) {
+      s256_line<false, true>(x, s, g.l, g.t);
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        x[a] = cscale(x[a], S256_INV_NN);
+        chi[g.e(a)] = x[a];
+      }
+    }
     float numO = 0.f, denO = 0.f, numP = 0.f, denP = 0.f;
 #pragma unroll
     for (int a = 0; a < 16; ++a) {
-      const float2 cv = chi[g.e(a)];
+      const float2 cv = x[a];
       const float2 o = obj_at(obj_i, c, g.y(a), g.x(a));
       if (upd_obj) {
         const float2 ps = psi_i[g.e(a)];
@@ -441,17 +466,32 @@ __global__ void __launch_bounds__(512) s256_cols_k(phx_group_desc d, int i, int 
     }
   } else if constexpr (OP == C_MODULUS) {
     // chi_hat = (sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask F (noise_models.py:47-63), then
-    // its inverse column transform into chi_a
+    // its inverse column transform into chi_a.  flags bit 0: the error sum of the slot's
+    // pattern (solvers.py:237), this tile's part, from the mode-0 plane into red.err
     const float2* F = (const float2*)d.farfield + plane;
     const float* D = d.patterns_s + (size_t)d.group_pos[b] * S256_NN;
     const float off = d.params[P_OFFSET], eps = d.params[P_EPS];
     float2* chi = (float2*)d.chi_a + plane;
+    float e2 = 0.f;
+    auto body = [&](auto isum) __attribute__((always_inline)) {
 #pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const int e = g.e(a);
-      float t;
-      const float u = farfield_factor<FM_MODULUS>(model_intensity(d, b, e), D[e], d.mask_s[e], off, eps, 0.f, 1.f, t);
-      x[a] = cscale(F[e], u);
+      for (int a = 0; a < 16; ++a) {
+        const int e = g.e(a);
+        float t;
+        const float u = farfield_factor<FM_MODULUS>(isum(e), D[e], d.mask_s[e], off, eps, 0.f, 1.f, t);
+        e2 += t;
+        x[a] = cscale(F[e], u);
+      }
+    };
+    if (M <= 4) {
+      const ModeSum4 ms(d, b);
+      body([&](int e) { return ms(e); });
+    } else {
+      body([&](int e) { return model_intensity(d, b, e); });
+    }
+    if ((flags & 1) && m == 0) {
+      const double tot = s256_block_sum<true>(e2, s);
+      if (threadIdx.x == 0) d.red[red_layout(d.n_slices, d.bmax, M).err + 8 * b + tile] = tot;
     }
     s256_line<true, true>(x, s, g.l, g.t);
 #pragma unroll
@@ -467,13 +507,21 @@ __global__ void __launch_bounds__(512) s256_cols_k(phx_group_desc d, int i, int 
     const float gscale = d.params[P_LOSS_SCALE];
     float2* chi = (float2*)d.chi_a + plane;
     float e2 = 0.f;
+    auto body = [&](auto isum) __attribute__((always_inline)) {
 #pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const int e = g.e(a);
-      float t;
-      const float u = farfield_factor<FM>(model_intensity(d, b, e), D[e], d.mask_s[e], off, eps, var, gscale, t);
-      e2 += t;
-      x[a] = cscale(F[e], u);
+      for (int a = 0; a < 16; ++a) {
+        const int e = g.e(a);
+        float t;
+        const float u = farfield_factor<FM>(isum(e), D[e], d.mask_s[e], off, eps, var, gscale, t);
+        e2 += t;
+        x[a] = cscale(F[e], u);
+      }
+    };
+    if (M <= 4) {
+      const ModeSum4 ms(d, b);
+      body([&](int e) { return ms(e); });
+    } else {
+      body([&](int e) { return model_intensity(d, b, e); });
     }
     if (m == 0) add_group_loss<FM>(d, e2, s.red);
     s256_line<true, true>(x, s, g.l, g.t);
@@ -541,21 +589,15 @@ __global__ void __launch_bounds__(512) s256_cols_k(phx_group_desc d, int i, int 
   }
 }
 
-// errors[b] = sqrt(nansum((I - D)^2)) over the slot's pattern (solvers.py:237; unmasked,
-// FP64 block sum).  One workgroup per slot.
-__global__ void __launch_bounds__(256) s256_errors_k(phx_group_desc d) {
-  __shared__ double scratch[4];
-  const int b = blockIdx.x;
-  if (!slot_active(d, b)) return;
-  const float* D = d.patterns_s + (size_t)d.group_pos[b] * S256_NN;
+// errors[b] = sqrt(nansum((I - D)^2)) over the slot's pattern (solvers.py:237; unmasked):
+// the 8 column tiles' FP64 partials that C_MODULUS left in red.err, added in tile order.
+__global__ void __launch_bounds__(64) s256_errors_k(phx_group_desc d) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= d.bmax || !slot_active(d, b)) return;
+  const double* part = d.red + red_layout(d.n_slices, d.bmax, d.n_modes).err + 8 * b;
   double e2 = 0.0;
-  for (int e = threadIdx.x; e < S256_NN; e += 256) {
-    const float df = model_intensity(d, b, e) - D[e];
-    const float t = df * df;
-    e2 += isnan(t) ? 0.0 : (double)t;
-  }
-  e2 = block_sum_f64(e2, scratch);
-  if (threadIdx.x == 0) d.errors[b] = (float)sqrt(e2);
+  for (int t = 0; t < S256_COL_TILES; ++t) e2 += part[t];
+  d.errors[b] = (float)sqrt(e2);
 }
 
 // ----- host launchers (n = 256) -----
@@ -569,6 +611,8 @@ inline dim3 s256_col_grid(const phx_group_desc& d, bool per_mode) {
   hipLaunchKernelGGL((s256_rows_k<OP>), s256_row_grid(d, PM), dim3(256), 0, stream, d, (I), (F))
 #define S256_COLS(OP, PM, I, F) \
   hipLaunchKernelGGL((s256_cols_k<OP>), s256_col_grid(d, PM), dim3(512), 0, stream, d, (I), (F))
+#define S256_ERRORS() \
+  hipLaunchKernelGGL(s256_errors_k, dim3((unsigned)((d.bmax + 63) / 64)), dim3(64), 0, stream, d)
 
 inline int s256_probe_spectra(const phx_group_desc& d, hipStream_t stream) {
   S256_ROWS(R_PROBE_FWD, true, 0, 0);
@@ -607,9 +651,59 @@ inline int s256_exit_spectra(const phx_group_desc& d, hipStream_t stream) {
 inline int s256_farfield(const phx_group_desc& d, int calc_error, hipStream_t stream) {
   S256_ROWS(R_EXIT, false, 0, 0);
   S256_COLS(C_EXIT, false, 0, 0);
-  if (calc_error) hipLaunchKernelGGL(s256_errors_k, dim3((unsigned)d.bmax), dim3(256), 0, stream, d);
-  S256_COLS(C_MODULUS, false, 0, 0);
+  S256_COLS(C_MODULUS, false, 0, calc_error ? 1 : 0);
+  if (calc_error) S256_ERRORS();
   S256_ROWS(R_CHI_FINISH, false, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// The forward model of a group with each slice step's finishing row pass folded into the
+// next step's first one: psi_0, then per slice a column pass (propagator) and one row pass
+// (finish psi_{i+1}, store it, transmit, forward rows), then the exit spectra: 2S + 1
+// passes instead of 3S + 1.
+inline int s256_group_forward(const phx_group_desc& d, hipStream_t stream) {
+  const int S = d.n_slices;
+  S256_COLS(C_SHIFT, false, 0, 0);
+  if (S == 1) {
+    S256_ROWS(R_SHIFT_FINISH, false, 0, 0);
+    S256_ROWS(R_EXIT, false, 0, 0);
+  } else {
+    S256_ROWS(R_SHIFT_FINISH, false, 0, 1);
+    for (int i = 0; i < S - 1; ++i) {
+      S256_COLS(C_PROPAGATE, false, i, 0);
+      S256_ROWS(R_SLICE_START, false, i + 1, 8);  // finish psi_{i+1}, then its step
+    }
+  }
+  S256_COLS(C_EXIT, false, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// The far-field modulus update alone: chi_a = ifft2(u F), errors when calc_error.
+inline int s256_farfield_update(const phx_group_desc& d, int calc_error, hipStream_t stream) {
+  S256_COLS(C_MODULUS, false, 0, calc_error ? 1 : 0);
+  if (calc_error) S256_ERRORS();
+  S256_ROWS(R_CHI_FINISH, false, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// The LSQ-ML backward sweep with each step's finishing row pass folded into the next
+// step's first one: the modulus column pass, then per slice one row pass (finish chi_i,
+// store it, the step-size sums, dP, forward rows) and a column pass (conj propagator),
+// and at slice 0 the probe spectra: 2S + 1 passes (+1 with a probe update) instead of 3S + 2.
+inline int s256_lsqml_backward_sweep(const phx_group_desc& d, int upd_obj, int upd_probe, int calc_error,
+                                     hipStream_t stream) {
+  const int S = d.n_slices;
+  S256_COLS(C_MODULUS, false, 0, calc_error ? 1 : 0);
+  if (calc_error) S256_ERRORS();
+  for (int i = S - 1; i >= 0; --i) {
+    const bool probe = i == 0 && upd_probe;
+    S256_ROWS(R_BACK, false, i, (upd_obj ? 1 : 0) | (probe ? 2 : 0) | 8);
+    if (i > 0) S256_COLS(C_PROPAGATE, false, i - 1, 1);
+    else if (probe) S256_COLS(C_BACK_PROBE, false, 0, 0);
+  }
   PHX_CHECK_LAUNCH();
   return PHX_OK;
 }
@@ -652,8 +746,17 @@ inline int s256_epie_probe_apply(const phx_group_desc& d, hipStream_t stream) {
   return PHX_OK;
 }
 
-// Gradient engine: exit spectra, then the loss gradient w.r.t. the exit wave into chi_a
-// and the group loss
+// Gradient engine: the loss gradient w.r.t. the exit wave into chi_a and the group loss,
+// from the exit spectra a forward launch left
+inline int s256_farfield_gradient_update(const phx_group_desc& d, int poisson, hipStream_t stream) {
+  if (poisson) S256_COLS(C_GRAD_POISSON, false, 0, 0);
+  else S256_COLS(C_GRAD_AMPLITUDE, false, 0, 0);
+  S256_ROWS(R_CHI_FINISH, false, 0, 0);
+  PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+// The same after the exit wave's own passes
 inline int s256_farfield_gradient(const phx_group_desc& d, int poisson, hipStream_t stream) {
   S256_ROWS(R_EXIT, false, 0, 0);
   S256_COLS(C_EXIT, false, 0, 0);
@@ -710,3 +813,4 @@ inline int s256_lsqml_probe_apply(const phx_group_desc& d, hipStream_t stream) {
 
 #undef S256_ROWS
 #undef S256_COLS
+#undef S256_ERRORS

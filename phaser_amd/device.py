@@ -206,8 +206,11 @@ class GroupWorkspace:
         # 192 x 192 plane (48 points per lane at 768 threads, 3 waves per SIMD) leaves too
         # few registers for a sweep that also carries its cutout and propagator loads
         # across slices: those kernels spill, and the per-slice kernels (which do not)
-        # run a slice step 1.4-2.5x faster, so planes wider than FUSED_MAX_N use them
-        self.fused = bool(fused) and n <= FUSED_MAX_N
+        # run a slice step 1.4-2.5x faster, so planes wider than FUSED_MAX_N use them.
+        # At SPLIT_N the "whole-sweep" entry points are launch sequences of split passes
+        # that fold each slice step's last row pass into the next step's first (2 passes
+        # per step instead of 3)
+        self.fused = bool(fused) and (n <= FUSED_MAX_N or n == SPLIT_N)
         # a second stream (default priority) for the probe-side kernels of a group step
         self.side_stream = torch.cuda.Stream(device=dev)
         self.concurrent = True
@@ -404,9 +407,19 @@ class GroupWorkspace:
             c("phx_group_forward", d, st)
             if update_probe:
                 c("phx_group_obj_mag", d, sh)
-            c("phx_lsqml_backward_sweep", d, int(update_object), int(update_probe), int(calc_error), st)
-            if update_position:
-                self._launch_position(d, st)
+            if update_position and self.n == SPLIT_N:
+                # split planes: the position step's work plane is the farfield slot, which
+                # slice 0's backward step then fills with the probe spectra, so the
+                # backward runs per slice with the position step before slice 0
+                c("phx_farfield_update", d, int(calc_error), st)
+                for i in range(self.S - 1, -1, -1):
+                    if i == 0:
+                        self._launch_position(d, st)
+                    c("phx_lsqml_backward", d, i, int(update_object), int(update_probe), st)
+            else:
+                c("phx_lsqml_backward_sweep", d, int(update_object), int(update_probe), int(calc_error), st)
+                if update_position:
+                    self._launch_position(d, st)
             if side is not None:
                 side.wait_stream(main)
             self._begun = None

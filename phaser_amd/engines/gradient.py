@@ -67,6 +67,9 @@ class MultisliceModel:
         self.fused = ok if fused is None else (bool(fused) and ok)
         if ws.n > FUSED_MAX_N:
             self.fused = False   # per-slice kernels for wide planes (see GroupWorkspace)
+        # split planes: the forward model as one launch sequence (phx_group_forward folds
+        # each slice step's last row pass into the next step's first); the adjoint per slice
+        self.split_forward = ws.n == SPLIT_N and ws.psi_slots >= ws.S
         # per-slice adjoint with every slice's wave gradient kept (chi_slices): the object
         # gradient is gathered per tile afterwards, as in the fused path, not scattered
         self.gather = ws.psi_slots >= ws.S and (ws.S == 1 or ws.chi_slices is not None)
@@ -78,7 +81,7 @@ class MultisliceModel:
         self._begun = None
         ws.fill_tilt(g, st)
         _lib.call("phx_probe_spectra", d, st)
-        if self.fused:
+        if self.fused or self.split_forward:
             # the whole forward model in one launch; its stored intensity gives the sums
             _lib.call("phx_group_forward", d, st)
             _lib.call("phx_dry_run_stats_intensity", d, st)
@@ -111,6 +114,9 @@ class MultisliceModel:
                 self._farfield_pending = True
             else:
                 _lib.call("phx_farfield_gradient_update", d, self.poisson, st)
+        elif self.split_forward:
+            _lib.call("phx_group_forward", d, st)
+            _lib.call("phx_farfield_gradient_update", d, self.poisson, st)
         else:
             _lib.call("phx_shift_probes", d, 0, int(ws.S > 1), st)
             for i in range(1, ws.S - 1):

@@ -283,6 +283,26 @@ def test_n256_lsqml_split_passes(slices, modes, graphs):
            strict=slices > 1)
 
 
+def test_n256_folded_passes_equal_per_slice_launches():
+    """At 256^2 the group forward and backward sweeps fold each slice step's finishing
+    row pass into the next step's first: the same arithmetic as the per-slice launches
+    (the finished wave is stored and used from registers instead of re-read), so LSQ-ML
+    with probe updates gives the same bits either way."""
+    from phaser_amd.engines.conventional import run_engine
+    from phaser_amd.plan import ConventionalEnginePlan, SOLVERS
+    _, state, pats = _problem(n=256, n_modes=2, n_slices=3, scan_shape=(3, 3), scan_step_px=10.0,
+                              jitter_px=0.5, seed=17)
+    plan = ConventionalEnginePlan(noise_model='amplitude', solver=SOLVERS['lsqml'](beta_object=0.5, beta_probe=0.5),
+                                  niter=2, grouping=5)
+    args = {'data': pats, 'dtype': numpy.float32, 'seed': 7, 'observer': None, 'recons_name': 't'}
+    a = run_engine(dict(args, state=state.copy()), plan, use_graphs=False).to_numpy()
+    b = run_engine(dict(args, state=state.copy()), plan, use_graphs=False, fused=False).to_numpy()
+    assert numpy.isfinite(a.object.data).all()
+    assert numpy.array_equal(a.object.data, b.object.data)
+    assert numpy.array_equal(a.probe.data, b.probe.data)
+    numpy.testing.assert_array_equal(a.progress.detector_errors, b.progress.detector_errors)
+
+
 def test_n256_epie_split_passes():
     """ePIE at 256^2 (split row and column launches, object update gathered per tile):
     three slices, two modes, two iterations against the oracle."""
