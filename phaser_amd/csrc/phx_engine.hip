@@ -933,11 +933,19 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   }
 }
 
-constexpr int UPD_TW = 64, UPD_TH = 4;
+// Object tiles of the gathers: one row of UPD_TW pixels (a multiple of 64, so each wave
+// owns 64 contiguous pixels of one row).  A wave reads 512 contiguous bytes of a plane
+// row per listed position and the block's waves read adjacent spans, so only the block's
+// two outer 128-byte lines are partial (a 64 x 4 tile had one partial line per wave
+// and row: 1.21x the algorithmic bytes).
+#ifndef PHX_UPD_TW
+#define PHX_UPD_TW 256
+#endif
+constexpr int UPD_TW = PHX_UPD_TW, UPD_TH = 256 / PHX_UPD_TW;
 constexpr int UPD_LIST = 256;  // positions listed per pass
 
 // The gather shared by the LSQ-ML object update and the gradient engine's object
-// gradient: for the pixel (y, x) of this thread in the 64 x 4 tile at (ty0, tx0) of
+// gradient: for the pixel (y, x) of this thread in the UPD_TW x UPD_TH tile at (ty0, tx0) of
 // slice i, dO = sum_{b,m} chi_i conj(psi_i) and (with WANT_G) g = sum_{b,m} |psi_i|^2
 // over the group's cutouts that cover it, positions in group order, modes in order.
 // PER_POS_W (ePIE) weights position b's term by 1 / aux[bmax + b] (its max |psi|^2).
@@ -984,10 +992,17 @@ __device__ __forceinline__ void gather_tile(const phx_group_desc& d, int i, int 
     }
     __syncthreads();
     const int cnt = s_cnt;
+    // this wave's pixels: row y, columns [wx0, wx0 + 64)
+    const int wx0 = x - lane;
+    const bool wave_out = y >= d.obj_h || wx0 >= d.obj_w;
 #pragma unroll 2
     for (int t = 0; t < cnt; ++t) {
       const int b = s_b[t];
       const int ly = y - s_y0[t], lx = x - s_x0[t];
+      // a listed cutout that misses this wave's row or its 64 columns (or a wave wholly
+      // outside the object) adds nothing: the wave skips its loads (wave-uniform test)
+      const int lxw = wx0 - s_x0[t];
+      if (wave_out || (unsigned)ly >= (unsigned)n || lxw >= n || lxw + 64 <= 0) continue;
       // branch-free: clamped (valid) address, contribution weighted by the in-cutout test
       const float w = ((unsigned)ly < (unsigned)n && (unsigned)lx < (unsigned)n) ? 1.f : 0.f;
       const size_t off = (size_t)min(max(ly, 0), n - 1) * n + min(max(lx, 0), n - 1);
@@ -1053,9 +1068,8 @@ __device__ __forceinline__ void clear_span(const ClearSpan& z) {
 // LSQ-ML object update of slice i (solvers.py:213-215, 248-257):
 //   O_i += beta_O (sum_b alpha_O,b) dO/(gpm + reg) * gpm/(gpm + 1e-16),
 //   dO  = sum_{b,m} scatter(chi_i conj(psi_i)),   gpm = sum_{b,m} scatter(|psi_i|^2).
-// One workgroup owns a 64 x 4 object tile of slice first + blockIdx.z (one pixel per
-// thread; a wave reads 512 contiguous bytes of a plane row, so the arbitrary offset of a
-// cutout inside its 128-byte lines costs 1.25x, not the 2x of 16-pixel rows).  Wave 0
+// One workgroup owns a UPD_TW x UPD_TH object tile of slice first + blockIdx.z (one pixel
+// per thread; see UPD_TW for the line alignment).  Wave 0
 // lists the group's positions whose cutout meets the tile, in group order (ballot +
 // prefix count); then every thread gathers both sums from those positions' chi_i and
 // psi_i planes (gather_tile): no atomics, and the result does not depend on scheduling.
@@ -1403,7 +1417,7 @@ __global__ void __launch_bounds__(T) epie_backward_k(phx_group_desc d, int i, in
 
 // ePIE object update of slice i (solvers.py:466-477):
 //   O_i += beta_O / S * sum_b (sum_m chi_i conj(psi_i)) / max_pixels(sum_m |psi_i|^2)_b
-// over the group's cutouts, gathered per 64 x 4 tile from the chi_i and psi_i planes in
+// over the group's cutouts, gathered per object tile from the chi_i and psi_i planes in
 // group order (gather_tile with the per-position weights epie_maxima_k left in aux):
 // no atomics, so the object does not depend on scheduling.  Runs after epie_backward_k
 // of slice i and before that of slice i-1 (which overwrites chi_i's buffer).  Pixels no
