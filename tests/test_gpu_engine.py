@@ -365,6 +365,23 @@ def test_lsqml_position_correction():
     assert e_g64 <= 3 * e_o64 + 1e-5
 
 
+def test_epie_position_correction():
+    """ePIE with momentum position correction against the oracle: ePIE's group step always
+    computes the position gradients (solvers.py:489-502; its run_iteration never forwards
+    update_position), and the engine loop applies them (run.py:95-110)."""
+    _, state, pats = _problem(n=32, n_modes=2, n_slices=2, scan_shape=(8, 8), scan_step_px=3.0, jitter_px=0.5,
+                              seed=5, defocus=30.0)
+    pos = {'step_size': 1e-2, 'max_step_size': 0.1, 'momentum': 0.9}
+    g, o32, o64 = _run_all(state, pats, solver='epie', niter=3, grouping=16, position=pos, beta=0.5)
+    _check(g, o32, o64)
+    d_g, d_o, d_64 = g.scan - state.scan, o32.scan - state.scan, o64.scan - state.scan
+    e_g64, e_o64 = rel_err(d_g, d_64), rel_err(d_o, d_64)
+    print(f"scan update: gpu-vs-fp32 {rel_err(d_g, d_o):.2e} gpu-vs-fp64 {e_g64:.2e} fp32-vs-fp64 {e_o64:.2e}",
+          flush=True)
+    assert numpy.abs(d_o).max() > 0
+    assert e_g64 <= 3 * e_o64 + 1e-5
+
+
 @pytest.mark.parametrize('solver,slices', [('lsqml', 3), ('epie', 3), ('lsqml', 5)])
 def test_fused_sweeps_and_deferred_updates_equal_per_slice_launches(solver, slices):
     """Whole-sweep kernels (one launch for the forward model, one for the LSQ-ML backward
