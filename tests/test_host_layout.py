@@ -48,3 +48,19 @@ def test_zero_column_waves_of_band_limited_propagators():
     # no band limit, or a single slice: nothing is skipped
     assert zero_column_waves(make_propagators(samp, 0.0197, numpy.array([20., 20.]), None), 128) == 0
     assert zero_column_waves(None, 128) == 0
+
+
+def test_split_plane_layout_is_natural_order():
+    """256^2 planes run in split row/column passes whose spectra stay in natural order:
+    the host's frequency map is the identity there (so patterns, mask, propagators and
+    the ky/kx tables upload unpermuted), and the twiddle table is W_256^j."""
+    from phaser_amd.optics import twiddles
+    f = scr_to_freq(256)
+    assert (f == numpy.arange(256)).all()
+    tw = twiddles(256)
+    numpy.testing.assert_allclose(tw, numpy.exp(-2j * numpy.pi * numpy.arange(256) / 256), atol=1e-7)
+    assert tw.dtype == numpy.complex64
+    # the other sides keep the lane-scrambled order (a permutation, not the identity)
+    for n in (16, 32, 64, 128, 192):
+        g = scr_to_freq(n)
+        assert sorted(g) == list(range(n)) and not (g == numpy.arange(n)).all()
