@@ -110,8 +110,9 @@ typedef struct phx_group_desc {
   void* chi_a;               /* (bmax, M, n, n) complex64 */
   void* chi_b;               /* (bmax, M, n, n) complex64 */
   void* farfield;            /* (bmax, M, n*n) complex64, scrambled exit-wave spectra */
-  float* intensity;          /* (bmax, M, n*n) float32 per-mode model intensity |F_m|^2 / n^2,
-                                scrambled; consumers add the M terms in mode order */
+  float* intensity;          /* (bmax, n*n, M) float32 per-mode model intensity |F_m|^2 / n^2,
+                                scrambled, the M terms of a point side by side; consumers
+                                add them in mode order */
   float* gpm;                /* (S, H, W) float32 group illumination */
   void* d_obj;               /* (H, W) complex64 object update accumulator (kept zero) */
   void* d_probe;             /* (M, n*n) complex64 probe update spectrum (kept zero) */
@@ -175,7 +176,7 @@ int phx_shift_probes(const phx_group_desc* d, int illum, int propagate, hipStrea
  * illumination of slice i when illum != 0 (solvers.py:210-222). */
 int phx_forward_slice(const phx_group_desc* d, int slice, int illum, hipStream_t stream);
 
-/* Exit wave (solvers.py:231-240): farfield = FFT(psi[S-1] * O_{S-1}), intensity[b][m] =
+/* Exit wave (solvers.py:231-240): farfield = FFT(psi[S-1] * O_{S-1}), intensity[b][e][m] =
  * |fft2|^2 / n^2 per mode (summed over m in mode order where read); illumination of slice S-1 when illum != 0.  Then the amplitude/Anscombe
  * modulus constraint (noise_models.py:47-63) chi_a = ifft2(update * Psi) and, when
  * calc_error != 0, errors[b] = sqrt(nansum((I - D)^2)) (solvers.py:237, FP64 sum). */
@@ -184,8 +185,8 @@ int phx_farfield(const phx_group_desc* d, int illum, int calc_error, hipStream_t
 /* Fused forward model of a group (simulation.py:241-285, solvers.py:202-236): one launch
  * carries each (slot, mode) plane through every slice with the wave in registers:
  * psi[0] = ifft2(fft2(probe) * shift_filter(b)), psi[i+1] = ifft2(fft2(psi[i] * O_i) *
- * prop_i) stored for every i, then farfield = FFT(psi[S-1] * O_{S-1}) and intensity[b][m] =
- * |farfield|^2 / n^2 (plain stores, one plane per mode), plus the per-slot norm
+ * prop_i) stored for every i, then farfield = FFT(psi[S-1] * O_{S-1}) and intensity[b][e][m] =
+ * |farfield|^2 / n^2 (plain stores, one term per mode), plus the per-slot norm
  * sum_m |psi[0]|^2.  Replaces phx_shift_probes + phx_forward_slice + the first half of
  * phx_farfield.  Needs psi_slots covering every slice. */
 int phx_group_forward(const phx_group_desc* d, hipStream_t stream);

@@ -104,40 +104,42 @@ __device__ __forceinline__ PropSeq prop_seq(const phx_group_desc& d, int b) {
 }
 
 // Model intensity sum_m |F_m|^2 / n^2 at spectrum point e of slot b.  The forward kernels
-// leave each mode's term in its own plane (intensity[b][m]) with plain stores, and the
-// terms are added here in mode order, as numpy's sum over the mode axis adds them: the
-// result does not depend on the order in which the modes' workgroups ran.
+// leave each mode's term with a plain store at intensity[(b n^2 + e) M + m] (the M terms of
+// a point side by side), and the terms are added here in mode order, as numpy's sum over
+// the mode axis adds them: the result does not depend on the order in which the modes'
+// workgroups ran.
+__device__ __forceinline__ float* intensity_term(const phx_group_desc& d, int b, int e, int m) {
+  return d.intensity + ((size_t)b * d.n * d.n + e) * d.n_modes + m;
+}
 __device__ __forceinline__ float model_intensity(const phx_group_desc& d, int b, int e) {
-  const size_t NN = (size_t)d.n * d.n;
-  const float* I = d.intensity + (size_t)b * d.n_modes * NN + e;
+  const float* I = intensity_term(d, b, e, 0);
   float s = I[0];
-  for (int k = 1; k < d.n_modes; ++k) s += I[k * NN];
+  for (int k = 1; k < d.n_modes; ++k) s += I[k];
   return s;
 }
 
-// model_intensity for up to four modes, as a per-point functor whose four loads (modes
-// past M re-read mode 0 and are not added) go out with the caller's other loads of the
-// point: no separate round trip, and the adds keep mode order.
+// model_intensity for up to four modes, as a per-point functor whose loads go out with
+// the caller's other loads of the point (no separate round trip): with four modes one
+// 16-byte load, otherwise up to four loads within the point's 16 bytes (a mode past M
+// re-reads mode 0 and is not added).  The adds keep mode order.
 struct ModeSum4 {
-  const float *i0, *i1, *i2, *i3;
-  bool h1, h2, h3;
+  const float* base;
+  int M;
   __device__ __forceinline__ ModeSum4(const phx_group_desc& d, int b) {
-    const size_t NN = (size_t)d.n * d.n;
-    const int M = d.n_modes;
-    i0 = d.intensity + (size_t)b * M * NN;
-    i1 = i0 + (M > 1 ? NN : 0);
-    i2 = i0 + (M > 2 ? 2 * NN : 0);
-    i3 = i0 + (M > 3 ? 3 * NN : 0);
-    h1 = M > 1;
-    h2 = M > 2;
-    h3 = M > 3;
+    base = intensity_term(d, b, 0, 0);
+    M = d.n_modes;
   }
   __device__ __forceinline__ float operator()(int e) const {
-    const float a0 = i0[e], a1 = i1[e], a2 = i2[e], a3 = i3[e];
+    if (M == 4) {
+      const float4 v = *reinterpret_cast<const float4*>(base + 4 * (size_t)e);
+      return ((v.x + v.y) + v.z) + v.w;
+    }
+    const float* I = base + (size_t)e * M;
+    const float a0 = I[0], a1 = I[M > 1 ? 1 : 0], a2 = I[M > 2 ? 2 : 0], a3 = I[M > 3 ? 3 : 0];
     float s = a0;
-    s = h1 ? s + a1 : s;
-    s = h2 ? s + a2 : s;
-    s = h3 ? s + a3 : s;
+    s = M > 1 ? s + a1 : s;
+    s = M > 2 ? s + a2 : s;
+    s = M > 3 ? s + a3 : s;
     return s;
   }
 };
@@ -147,15 +149,14 @@ struct ModeSum4 {
 template <int N>
 __device__ __forceinline__ void model_intensity_cols(const phx_group_desc& d, int b, int q, int rc,
                                                      float (&Iv)[Plane<N>::E]) {
-  constexpr int NN = N * N, E = Plane<N>::E, QL = Plane<N>::QL;
+  constexpr int E = Plane<N>::E, QL = Plane<N>::QL;
   const int M = d.n_modes;
-  const float* I = d.intensity + (size_t)b * M * NN + q * N + rc;
+  const float* I = intensity_term(d, b, q * N + rc, 0);
 #pragma unroll
-  for (int k = 0; k < E; ++k) Iv[k] = I[k * QL * N];
+  for (int k = 0; k < E; ++k) Iv[k] = I[(size_t)k * QL * N * M];
   for (int mm = 1; mm < M; ++mm) {
-    const float* Im = I + (size_t)mm * NN;
 #pragma unroll
-    for (int k = 0; k < E; ++k) Iv[k] += Im[k * QL * N];
+    for (int k = 0; k < E; ++k) Iv[k] += I[(size_t)k * QL * N * M + mm];
   }
 }
 
@@ -363,7 +364,7 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d) {
   const float2* obj_s = (const float2*)d.object + s * HW;
   float* gpm_s = d.gpm + s * HW;
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
-  float* I = d.intensity + ((size_t)b * M + m) * NN;  // this mode's term (model_intensity sums them)
+  float* I = intensity_term(d, b, 0, m);  // this mode's terms, stride M (model_intensity sums them)
   pl.forward(
       [&](int y, int x) {
         const float2 v = psi_s[y * N + x];
@@ -374,7 +375,7 @@ __global__ void __launch_bounds__(T) farfield_fwd_k(phx_group_desc d) {
       [&](int py, int px, float2& v) {
         const int e = py * N + px;
         F[e] = v;
-        I[e] = abs2f(v) * inv_nn;
+        I[(size_t)e * M] = abs2f(v) * inv_nn;
       });
 }
 
@@ -619,12 +620,12 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
   }
   pl.forward_regs(x);
   float2* F = (float2*)d.farfield + ((size_t)b * M + m) * NN;
-  float* I = d.intensity + ((size_t)b * M + m) * NN;  // this mode's term (model_intensity sums them)
+  float* I = intensity_term(d, b, 0, m);  // this mode's terms, stride M (model_intensity sums them)
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int e = (pl.QL * k + pl.q) * N + pl.rc;
     F[e] = x[k];
-    I[e] = abs2f(x[k]) * inv_nn;
+    I[(size_t)e * M] = abs2f(x[k]) * inv_nn;
   }
   const double tot = block_sum_f64(nrm, red_lds_);
   if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(S, d.bmax, M).norm + b], tot);

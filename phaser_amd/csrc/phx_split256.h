@@ -120,7 +120,7 @@ enum S256Op {
   C_PROBE_FWD,       // probe_k[m] forward columns in place                          (per mode)
   C_SHIFT,           // probe_k[m] filt_b -> inverse columns -> psi_0
   C_PROPAGATE,       // forward columns, * p_i (or conj(p_{i-1}) backward), inverse columns, in place
-  C_EXIT,            // forward columns -> farfield, intensity[b][m] = |F|^2 / n^2
+  C_EXIT,            // forward columns -> farfield, this mode's |F|^2 / n^2 intensity terms
   C_MODULUS,         // modulus update of farfield -> inverse columns -> chi_a
   C_BACK_PROBE,      // forward columns of dP, * conj(filt_b) -> farfield (per-position probe spectrum)
   C_PROBE_INV,       // d_probe[m] inverse columns in place                          (per mode)
@@ -455,14 +455,14 @@ __global__ void __launch_bounds__(512) s256_cols_k(phx_group_desc d, int i, int 
     for (int a = 0; a < 16; ++a) w[g.e(a)] = x[a];
   } else if constexpr (OP == C_EXIT) {
     float2* F = (float2*)d.farfield + plane;
-    float* I = d.intensity + plane;  // this mode's term (model_intensity sums them)
+    float* I = intensity_term(d, b, 0, m);  // this mode's terms, stride M (model_intensity sums them)
 #pragma unroll
     for (int a = 0; a < 16; ++a) x[a] = F[g.e(a)];
     s256_line<true, false>(x, s, g.l, g.t);
 #pragma unroll
     for (int a = 0; a < 16; ++a) {
       F[g.e(a)] = x[a];
-      I[g.e(a)] = abs2f(x[a]) * S256_INV_NN;
+      I[(size_t)g.e(a) * M] = abs2f(x[a]) * S256_INV_NN;
     }
   } else if constexpr (OP == C_MODULUS) {
     // chi_hat = (sqrt(D + c)/(sqrt(I + c) + eps) - 1) mask F (noise_models.py:47-63), then
@@ -640,7 +640,7 @@ inline int s256_forward_slice(const phx_group_desc& d, int i, hipStream_t stream
   return PHX_OK;
 }
 
-// farfield = FFT(psi[S-1] O_{S-1}) and intensity[b][m] = |farfield|^2 / n^2
+// farfield = FFT(psi[S-1] O_{S-1}) and each mode's |farfield|^2 / n^2 term of the intensity
 inline int s256_exit_spectra(const phx_group_desc& d, hipStream_t stream) {
   S256_ROWS(R_EXIT, false, 0, 0);
   S256_COLS(C_EXIT, false, 0, 0);

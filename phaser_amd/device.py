@@ -7,7 +7,7 @@ HBM layout (config with P positions, n x n planes, M modes, S slices, H x W slic
   patterns_s (P, n*n) f32 and mask_s (n*n) f32 are resident, in scrambled frequency order;
   props_s (S-1, n*n) c64 scrambled; psi (S, B, M, n, n) c64 holds every slice's forward
   wave for the backward pass; chi_a / chi_b (B, M, n, n) ping-pong the back-propagated
-  exit-wave update; farfield (B, M, n*n) c64 and intensity (B, M, n*n) f32 carry the exit
+  exit-wave update; farfield (B, M, n*n) c64 and intensity (B, n*n, M) f32 carry the exit
   spectra between the two far-field kernels; gpm (S, H, W) f32, d_obj (H, W) c64,
   d_probe (M, n*n) c64 are the scatter accumulators; red holds the FP64 step-size sums.
 Per-iteration tables: group_pos (G, B) int32, group_size (G,) int32, start (P, 2) int32,
@@ -176,9 +176,10 @@ class GroupWorkspace:
         self.chi_a = torch.empty((B, M, NN), dtype=c64, device=dev)
         self.chi_b = torch.empty((B, M, NN), dtype=c64, device=dev)
         self.farfield = torch.empty((B, M, NN), dtype=c64, device=dev)
-        # one plane per mode: the forward kernels store |F_m|^2 / n^2 and the readers add
-        # the modes in order (no atomics: the model intensity is the same on every run)
-        self.intensity = torch.empty((B, self.M, NN), dtype=f32, device=dev)
+        # one term per mode and point, the M terms of a point side by side: the forward
+        # kernels store |F_m|^2 / n^2 and the readers add the modes in order (no atomics:
+        # the model intensity is the same on every run)
+        self.intensity = torch.empty((B, NN, self.M), dtype=f32, device=dev)
         self.gpm = torch.empty((S, H, W), dtype=f32, device=dev)
         self.d_obj = torch.zeros((H, W), dtype=c64, device=dev)
         self.d_probe = torch.zeros((M, NN), dtype=c64, device=dev)
