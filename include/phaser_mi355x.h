@@ -114,7 +114,7 @@ typedef struct phx_group_desc {
                                 scrambled, the M terms of a point side by side; consumers
                                 add them in mode order */
   float* gpm;                /* (S, H, W) float32 group illumination */
-  void* d_obj;               /* (H, W) complex64 object update accumulator (kept zero) */
+  void* d_obj;               /* unused (object updates are gathered per tile); may be NULL */
   void* d_probe;             /* (M, n*n) complex64 probe update spectrum (kept zero) */
   void* probe_k;             /* (M, n*n) complex64 probe spectra, scrambled */
   float* gom;                /* (n*n) float32 group object magnitude */

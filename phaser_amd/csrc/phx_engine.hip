@@ -1886,8 +1886,7 @@ int phx_lsqml_backward(const phx_group_desc* d, int slice, int update_object, in
                        hipStream_t stream) {
   int r = check_desc(d);
   if (r) return r;
-  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (update_object && !d->d_obj) ||
-      (update_probe && !d->d_probe))
+  if (slice < 0 || slice >= d->n_slices || !d->chi_a || !d->chi_b || (update_probe && !d->d_probe))
     return PHX_EINVAL;
   if (d->n == S256) return s256_lsqml_backward(*d, slice, update_object, update_probe, stream);
 #define L(N, T) launch_lsqml_backward<N, T>(*d, slice, update_object, update_probe, stream)

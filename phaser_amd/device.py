@@ -8,8 +8,9 @@ HBM layout (config with P positions, n x n planes, M modes, S slices, H x W slic
   props_s (S-1, n*n) c64 scrambled; psi (S, B, M, n, n) c64 holds every slice's forward
   wave for the backward pass; chi_a / chi_b (B, M, n, n) ping-pong the back-propagated
   exit-wave update; farfield (B, M, n*n) c64 and intensity (B, n*n, M) f32 carry the exit
-  spectra between the two far-field kernels; gpm (S, H, W) f32, d_obj (H, W) c64,
-  d_probe (M, n*n) c64 are the scatter accumulators; red holds the FP64 step-size sums.
+  spectra between the two far-field kernels; gpm (S, H, W) f32 (illumination scatter of the
+  per-slice entry points, unused by the engines) and d_probe (M, n*n) c64 are accumulators;
+  red holds the FP64 step-size sums.
 Per-iteration tables: group_pos (G, B) int32, group_size (G,) int32, start (P, 2) int32,
 shift (P, 2) f32.  Nothing is allocated after construction, so a whole iteration can be
 captured in one HIP graph and replayed.
@@ -181,7 +182,6 @@ class GroupWorkspace:
         # the model intensity is the same on every run)
         self.intensity = torch.empty((B, NN, self.M), dtype=f32, device=dev)
         self.gpm = torch.empty((S, H, W), dtype=f32, device=dev)
-        self.d_obj = torch.zeros((H, W), dtype=c64, device=dev)
         self.d_probe = torch.zeros((M, NN), dtype=c64, device=dev)
         self.probe_k = torch.empty((M, NN), dtype=c64, device=dev)
         # two object-magnitude planes, alternating by group slot: the clear of group g+1's
@@ -239,7 +239,7 @@ class GroupWorkspace:
         d.patterns_s, d.mask_s, d.params = self.patterns_s.data_ptr(), self.mask_s.data_ptr(), self.params.data_ptr()
         d.psi, d.chi_a, d.chi_b = self.psi.data_ptr(), self.chi_a.data_ptr(), self.chi_b.data_ptr()
         d.farfield, d.intensity, d.gpm = self.farfield.data_ptr(), self.intensity.data_ptr(), self.gpm.data_ptr()
-        d.d_obj, d.d_probe, d.probe_k = self.d_obj.data_ptr(), self.d_probe.data_ptr(), self.probe_k.data_ptr()
+        d.d_obj, d.d_probe, d.probe_k = None, self.d_probe.data_ptr(), self.probe_k.data_ptr()
         d.gom = self.gom[g % 2].data_ptr()
         d.red = self.red[g].data_ptr()
         d.errors = self.errors[g].data_ptr()
