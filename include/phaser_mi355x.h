@@ -314,6 +314,15 @@ int phx_gradient_sweep(const phx_group_desc* d, int want_object, int want_probe,
  * want_object).  For callers that need the loss only after the adjoint pass. */
 int phx_gradient_sweep_farfield(const phx_group_desc* d, int poisson, int want_object, int want_probe,
                                 int want_positions, hipStream_t stream);
+/* phx_group_forward followed by phx_gradient_sweep_farfield (the same values; the loss
+ * and the position gradients are FP64 sums with atomics), the loss and gradient of a
+ * group (gradient/run.py:293-349).  At 128^2 with more planes (bmax * n_modes) than the
+ * device has CUs, the two sweeps run as plane tasks in launches of one workgroup per
+ * CU: a launch holds the remaining forward sweeps, then adjoint sweeps of slots whose
+ * forward sweeps all ran in an earlier launch (384 planes on 256 CUs: 3 launches for
+ * the 4 rounds of the two sweeps).  Otherwise it launches the two sweeps. */
+int phx_gradient_group_sweeps(const phx_group_desc* d, int poisson, int want_object, int want_probe,
+                              int want_positions, hipStream_t stream);
 /* grad_obj[i] = sum_{b,m} scatter(g_i conj(psi_i)) for slices first .. first+count-1
  * from the wave gradients phx_gradient_sweep stored, gathered per object tile in group
  * and mode order (no atomics; deterministic).  Writes every pixel of those slices (zero

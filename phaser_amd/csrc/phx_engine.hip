@@ -11,6 +11,7 @@
 #include "phx_fft.h"
 #include "../../include/phaser_mi355x.h"
 
+#include <algorithm>
 #include <climits>
 #include <mutex>
 
@@ -571,12 +572,10 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
 //   psi_0 = ifft2(fft2(P_m) filt_b), psi_{i+1} = ifft2(fft2(psi_i O_i) p_i),
 //   farfield = FFT(psi_{S-1} O_{S-1}) (unnormalised), intensity += |farfield|^2 / n^2,
 // storing every psi_i and the norm sum_m |psi_0|^2 of each slot.
+// The body runs plane (b, m) on the workgroup's plane and reduction LDS.
 template <int N, int T>
-__global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
-  int b, m;
-  sweep_plane(d, b, m);
-  if (!slot_active(d, b)) return;
-  PHX_PLANE_SETUP(N, T);
+__device__ __forceinline__ void group_forward_plane(const phx_group_desc& d, int b, int m, Plane<N>& pl,
+                                                    double* red_lds_) {
   constexpr int NN = N * N;
   constexpr int E = Plane<N>::E;
   constexpr float inv_nn = 1.0f / (float)NN;
@@ -629,6 +628,15 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
   }
   const double tot = block_sum_f64(nrm, red_lds_);
   if (threadIdx.x == 0) atomic_add_f64(&d.red[red_layout(S, d.bmax, M).norm + b], tot);
+}
+
+template <int N, int T>
+__global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
+  int b, m;
+  sweep_plane(d, b, m);
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  group_forward_plane<N, T>(d, b, m, pl, red_lds_);
 }
 
 // LSQ-ML backward sweep of a group (solvers.py:231-271): the modulus update of the far
@@ -814,11 +822,8 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
 // group loss included) in the first column pass and transforms it in registers, and
 // stores slice S-1's wave gradient into chi_a.
 template <int N, int T, bool WANT_OBJ, bool WANT_P0, int FF>
-__global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want_probe, int want_pos) {
-  int b, m;
-  sweep_plane(d, b, m);
-  if (!slot_active(d, b)) return;
-  PHX_PLANE_SETUP(N, T);
+__device__ __forceinline__ void gradient_sweep_plane(const phx_group_desc& d, int b, int m, int want_probe,
+                                                     int want_pos, Plane<N>& pl, double* red_lds_) {
   constexpr int NN = N * N;
   constexpr int E = Plane<N>::E;
   constexpr int G = E < 8 ? E : 8;  // points per load group
@@ -932,6 +937,39 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
       atomic_add_f64(&d.red[L.pos_num + 2 * b + 1], tx * (double)inv_nn);
     }
   }
+}
+
+template <int N, int T, bool WANT_OBJ, bool WANT_P0, int FF>
+__global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want_probe, int want_pos) {
+  int b, m;
+  sweep_plane(d, b, m);
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  gradient_sweep_plane<N, T, WANT_OBJ, WANT_P0, FF>(d, b, m, want_probe, want_pos, pl, red_lds_);
+}
+
+// A group's forward and adjoint sweeps as plane tasks in launches of at most one
+// workgroup per CU (phx_gradient_group_sweeps): task t < n_fwd is the forward sweep of
+// plane fwd_lo + t, the others the adjoint sweep (with the far-field step FF) of plane
+// adj_lo + t - n_fwd, where plane q is (b, m) = (q / M, q mod M).  The host places an
+// adjoint plane only in a launch after the one holding the forward sweeps of all its
+// slot's modes.  Workgroups go to the XCDs round-robin by linear id, so with a multiple
+// of 8 of them XCD x takes the contiguous tasks [x n/8, (x+1) n/8): the modes of a slot
+// share one L2.
+template <int N, int T, bool WANT_OBJ, bool WANT_P0, int FF>
+__global__ void __launch_bounds__(T) gradient_tasks_k(phx_group_desc d, int want_probe, int want_pos, int fwd_lo,
+                                                      int n_fwd, int adj_lo) {
+  const int n = gridDim.x, L = blockIdx.x;
+  const int t = (n & 7) == 0 ? (L & 7) * (n >> 3) + (L >> 3) : L;
+  const bool fwd = t < n_fwd;
+  const int q = fwd ? fwd_lo + t : adj_lo + (t - n_fwd);
+  const int b = q / d.n_modes, m = q - b * d.n_modes;
+  if (!slot_active(d, b)) return;
+  PHX_PLANE_SETUP(N, T);
+  if (fwd)
+    group_forward_plane<N, T>(d, b, m, pl, red_lds_);
+  else
+    gradient_sweep_plane<N, T, WANT_OBJ, WANT_P0, FF>(d, b, m, want_probe, want_pos, pl, red_lds_);
 }
 
 // Object tiles of the gathers: one row of UPD_TW pixels (a multiple of 64, so each wave
@@ -1767,6 +1805,21 @@ int probe_strip_scratch_ptr(float2** out) {
   *out = (float2*)g_scratch_ptr[dev];
   return PHX_OK;
 }
+
+std::once_flag g_cu_once[SCRATCH_MAX_DEVICES];
+int g_cu_count[SCRATCH_MAX_DEVICES];
+
+// Compute units of the current device (queried once per device), or 0 if unknown.
+int device_cu_count() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES) return 0;
+  std::call_once(g_cu_once[dev], [dev] {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) c = 0;
+    g_cu_count[dev] = c;
+  });
+  return g_cu_count[dev];
+}
 }  // namespace
 
 extern "C" {
@@ -2036,6 +2089,66 @@ int phx_gradient_sweep_farfield(const phx_group_desc* d, int poisson, int want_o
 #undef KF
 #undef K
   PHX_CHECK_LAUNCH();
+  return PHX_OK;
+}
+
+int phx_gradient_group_sweeps(const phx_group_desc* d, int poisson, int want_object, int want_probe,
+                              int want_positions, hipStream_t stream) {
+  int r = check_desc(d);
+  if (r) return r;
+  if (!d->probe_k || !d->farfield || !d->intensity || !d->chi_a || !d->patterns_s || !d->mask_s ||
+      (d->n_slices > 1 && !d->props) || (want_object && d->n_slices > 1 && !d->chi_slices) ||
+      ((want_probe || want_positions) && (!d->d_probe || !d->probe_k)))
+    return PHX_EINVAL;
+  if (d->psi_slots > 0 && d->psi_slots < d->n_slices) return PHX_EINVAL;  // needs every psi_i
+  // Plane tasks: P forward sweeps, then P adjoint sweeps, in launches of at most C (one
+  // workgroup per CU at 128^2, so each launch is one round).  Launch k takes the
+  // remaining forward planes first, then adjoint planes whose slots' forward sweeps all
+  // ran in earlier launches.  For P = 384 planes on 256 CUs that is 3 full launches
+  // where the two sweeps take 4 rounds (2 of them half full).
+  const int P = d->bmax * d->n_modes, M = d->n_modes, C = device_cu_count();
+  int launches = 0;
+  if (d->n == 128 && C > 0 && P > C) {
+    for (int fwd = 0, adj = 0; adj < P; ++launches) {
+      const int nf = std::min(C, P - fwd);
+      const int na = std::min(C - nf, (fwd / M) * M - adj);
+      fwd += nf;
+      adj += std::max(na, 0);
+    }
+  }
+  if (launches == 0 || launches >= 2 * ((P + C - 1) / C)) {
+    r = phx_group_forward(d, stream);
+    if (r) return r;
+    return phx_gradient_sweep_farfield(d, poisson, want_object, want_probe, want_positions, stream);
+  }
+  const bool p0 = want_probe || want_positions;
+  for (int fwd = 0, adj = 0; adj < P;) {
+    const int nf = std::min(C, P - fwd);
+    const int na = std::max(0, std::min(C - nf, (fwd / M) * M - adj));
+#define K(WO, WP, FF)                                                                                         \
+  hipLaunchKernelGGL((gradient_tasks_k<128, 1024, WO, WP, FF>), dim3((unsigned)(nf + na)), dim3(1024), 0, stream, \
+                     *d, want_probe, want_positions, fwd, nf, adj)
+#define KF(WO, WP)                     \
+  do {                                 \
+    if (poisson)                       \
+      K(WO, WP, FM_GRAD_POISSON);      \
+    else                               \
+      K(WO, WP, FM_GRAD_AMPLITUDE);    \
+  } while (0)
+    if (want_object && p0)
+      KF(true, true);
+    else if (want_object)
+      KF(true, false);
+    else if (p0)
+      KF(false, true);
+    else
+      KF(false, false);
+#undef KF
+#undef K
+    PHX_CHECK_LAUNCH();
+    fwd += nf;
+    adj += na;
+  }
   return PHX_OK;
 }
 

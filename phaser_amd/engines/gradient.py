@@ -73,6 +73,11 @@ class MultisliceModel:
         # per-slice adjoint with every slice's wave gradient kept (chi_slices): the object
         # gradient is gathered per tile afterwards, as in the fused path, not scattered
         self.gather = ws.psi_slots >= ws.S and (ws.S == 1 or ws.chi_slices is not None)
+        # with the far-field step in the adjoint sweep, the forward sweep is launched with
+        # it (phx_gradient_group_sweeps schedules both as plane tasks when the group has
+        # more planes than the device has CUs)
+        self.group_sweeps = True
+        self._forward_pending = False
 
     def dry_run(self, g):
         """errors[b] = sum(D) / model intensity of every slot (gradient/run.py:408-438)."""
@@ -96,7 +101,8 @@ class MultisliceModel:
         loss lands in ws.red[g, loss].  probe_spectra=False reuses the probe spectra of
         the previous forward (valid while the probe is unchanged).  loss_in_backward=True
         (fused path) leaves the far-field gradient step and the loss to the adjoint sweep
-        of backward(), which computes them in its first pass."""
+        of backward(), which computes them in its first pass, and with group_sweeps the
+        forward sweep too, which backward() launches with the adjoint."""
         ws, d, st = self.ws, self.ws.desc_refs[g], stream_handle()
         if self._begun != g:
             _lib.call("phx_group_begin", d, st)
@@ -107,9 +113,13 @@ class MultisliceModel:
         # group's tilted propagators in place
         ws.fill_tilt(g, st)
         self._farfield_pending = False
+        self._forward_pending = False
         if self.fused:
             # every slice in one launch, then the far-field gradient and loss
-            _lib.call("phx_group_forward", d, st)
+            if loss_in_backward and self.group_sweeps:
+                self._forward_pending = True
+            else:
+                _lib.call("phx_group_forward", d, st)
             if loss_in_backward:
                 self._farfield_pending = True
             else:
@@ -138,7 +148,11 @@ class MultisliceModel:
         if self.fused:
             # the sweep stores each slice's wave gradient; one gather per object tile then
             # forms every slice's object gradient (all pixels written, no atomics)
-            if self._farfield_pending:
+            if self._forward_pending:
+                _lib.call("phx_gradient_group_sweeps", d, self.poisson, int(want_object), int(want_probe),
+                          int(want_positions), st)
+                self._forward_pending = self._farfield_pending = False
+            elif self._farfield_pending:
                 _lib.call("phx_gradient_sweep_farfield", d, self.poisson, int(want_object), int(want_probe),
                           int(want_positions), st)
                 self._farfield_pending = False

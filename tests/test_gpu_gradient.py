@@ -141,6 +141,44 @@ def test_farfield_step_in_sweep_equals_separate_launch(noise):
     assert abs(res[0][3] - res[1][3]) <= 1e-9 * abs(res[0][3])
 
 
+@pytest.mark.parametrize('noise', ['amplitude', 'poisson'])
+def test_group_sweeps_as_plane_tasks_equal_two_sweeps(noise):
+    """phx_gradient_group_sweeps on 64 positions x 6 modes = 384 planes of 128^2 (more
+    than the MI355X's 256 CUs, so the forward and adjoint sweeps run as plane tasks in 3
+    launches), then a 40-position group in the same slots (its inactive slots skipped),
+    against phx_group_forward + phx_gradient_sweep_farfield.  Every plane runs the same
+    code either way, so the stored waves, the intensity terms and the object and probe
+    gradients agree to the last bit; the loss and the position gradients are FP64 sums
+    with atomics."""
+    import torch
+    from phaser_amd.engines.gradient import make_model
+    from phaser_amd.plan import NOISE_MODELS
+    _cfg, state, pats = _problem(S=2, M=6, n=128, scan=(8, 13), noise=noise, seed=29)
+    idx = numpy.arange(8 * 13)
+    groups = [numpy.array(numpy.unravel_index(idx[:64], (8, 13))), numpy.array(numpy.unravel_index(idx[64:], (8, 13)))]
+    print(f"{torch.cuda.get_device_properties(0).multi_processor_count} CUs for 384 planes", flush=True)
+    res = []
+    for tasks in (False, True):
+        model, _ = make_model(state, pats.patterns, pats.pattern_mask, NOISE_MODELS[noise](), groups=groups)
+        assert model.fused and model.ws.B * model.ws.M == 384
+        model.group_sweeps = tasks
+        out = []
+        for g, gs in ((0, 64), (1, 40)):
+            model.forward(g, loss_in_backward=True)
+            model.backward(g)
+            ws = model.ws
+            out.append([t.cpu().numpy() for t in (ws.psi[:, :gs], ws.intensity[:gs], ws.grad_obj, ws.grad_probe,
+                                                  model.position_grads(g)[:gs])] + [float(model.loss_tensor(g))])
+        res.append(out)
+    for g in range(2):
+        a, b = res[0][g], res[1][g]
+        for name, u, v in zip(('psi', 'intensity', 'object gradient', 'probe gradient'), a[:4], b[:4]):
+            assert numpy.array_equal(u, v), f"group {g}: {name} differs"
+        e_pos, e_loss = rel_err(a[4], b[4]), abs(a[5] - b[5]) / abs(a[5])
+        print(f"group {g}: positions {e_pos:.2e}, loss {e_loss:.2e}", flush=True)
+        assert e_pos < 1e-9 and e_loss < 1e-12
+
+
 def test_dry_run_from_forward_intensity_equals_per_slice():
     """The gradient engine's dry-run ratios sum(D) / sum_m |psi_{S-1} O_{S-1}|^2 from one
     fused forward launch and its stored intensity (Parseval) against the per-slice
