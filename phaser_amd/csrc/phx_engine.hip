@@ -169,6 +169,17 @@ __device__ __forceinline__ void model_intensity_cols(const phx_group_desc& d, in
   load_twiddles<N>(lds_ + Plane<N>::LDS_FLOAT2, (const float2*)d.twiddle);       \
   __syncthreads();
 
+// The same in two halves, for kernels that run one of several plane bodies: the kernel
+// declares the LDS (one declaration shared by the bodies), the body binds the plane.
+#define PHX_PLANE_LDS(N, T)                                                      \
+  static_assert(T == Plane<N>::T, "plane kernels run one workgroup of Plane<N>::T threads per plane");               \
+  __shared__ float2 lds_[Plane<N>::LDS_FLOAT2 + N];                              \
+  [[maybe_unused]] __shared__ double red_lds_[T / 64];
+#define PHX_PLANE_BIND(N)                                                        \
+  Plane<N> pl(lds_, lds_ + Plane<N>::LDS_FLOAT2);                                \
+  load_twiddles<N>(lds_ + Plane<N>::LDS_FLOAT2, (const float2*)d.twiddle);       \
+  __syncthreads();
+
 __device__ __forceinline__ bool slot_active(const phx_group_desc& d, int b) { return b < *d.group_size; }
 
 // Tilted propagators of the group's slots (simulation.py:193-222): tprops[j][b][e] =
@@ -572,10 +583,15 @@ __global__ void __launch_bounds__(T) lsqml_backward_k(phx_group_desc d, int i) {
 //   psi_0 = ifft2(fft2(P_m) filt_b), psi_{i+1} = ifft2(fft2(psi_i O_i) p_i),
 //   farfield = FFT(psi_{S-1} O_{S-1}) (unnormalised), intensity += |farfield|^2 / n^2,
 // storing every psi_i and the norm sum_m |psi_0|^2 of each slot.
-// The body runs plane (b, m) on the workgroup's plane and reduction LDS.
+// The body runs plane (b, m) on the workgroup's plane and reduction LDS (PHX_PLANE_LDS).
+// At 128^2 it stays within 104 VGPRs: 4 waves per SIMD then leave 96, room for one
+// group_obj_mag_k workgroup (8 waves of 48 VGPRs, 2 per SIMD) beside the sweep, which the
+// LSQ-ML group step runs on a second stream.  At 112 VGPRs that workgroup waits for the
+// sweep's CUs and then delays the backward sweep's workgroups (2.5% of an iteration).
 template <int N, int T>
-__device__ __forceinline__ void group_forward_plane(const phx_group_desc& d, int b, int m, Plane<N>& pl,
+__device__ __forceinline__ void group_forward_plane(const phx_group_desc& d, int b, int m, float2* lds_,
                                                     double* red_lds_) {
+  PHX_PLANE_BIND(N);
   constexpr int NN = N * N;
   constexpr int E = Plane<N>::E;
   constexpr float inv_nn = 1.0f / (float)NN;
@@ -635,8 +651,8 @@ __global__ void __launch_bounds__(T) group_forward_k(phx_group_desc d) {
   int b, m;
   sweep_plane(d, b, m);
   if (!slot_active(d, b)) return;
-  PHX_PLANE_SETUP(N, T);
-  group_forward_plane<N, T>(d, b, m, pl, red_lds_);
+  PHX_PLANE_LDS(N, T);
+  group_forward_plane<N, T>(d, b, m, lds_, red_lds_);
 }
 
 // LSQ-ML backward sweep of a group (solvers.py:231-271): the modulus update of the far
@@ -823,7 +839,8 @@ __global__ void __launch_bounds__(T) lsqml_backward_fused_k(phx_group_desc d, in
 // stores slice S-1's wave gradient into chi_a.
 template <int N, int T, bool WANT_OBJ, bool WANT_P0, int FF>
 __device__ __forceinline__ void gradient_sweep_plane(const phx_group_desc& d, int b, int m, int want_probe,
-                                                     int want_pos, Plane<N>& pl, double* red_lds_) {
+                                                     int want_pos, float2* lds_, double* red_lds_) {
+  PHX_PLANE_BIND(N);
   constexpr int NN = N * N;
   constexpr int E = Plane<N>::E;
   constexpr int G = E < 8 ? E : 8;  // points per load group
@@ -944,8 +961,8 @@ __global__ void __launch_bounds__(T) gradient_sweep_k(phx_group_desc d, int want
   int b, m;
   sweep_plane(d, b, m);
   if (!slot_active(d, b)) return;
-  PHX_PLANE_SETUP(N, T);
-  gradient_sweep_plane<N, T, WANT_OBJ, WANT_P0, FF>(d, b, m, want_probe, want_pos, pl, red_lds_);
+  PHX_PLANE_LDS(N, T);
+  gradient_sweep_plane<N, T, WANT_OBJ, WANT_P0, FF>(d, b, m, want_probe, want_pos, lds_, red_lds_);
 }
 
 // A group's forward and adjoint sweeps as plane tasks in launches of at most one
@@ -965,11 +982,11 @@ __global__ void __launch_bounds__(T) gradient_tasks_k(phx_group_desc d, int want
   const int q = fwd ? fwd_lo + t : adj_lo + (t - n_fwd);
   const int b = q / d.n_modes, m = q - b * d.n_modes;
   if (!slot_active(d, b)) return;
-  PHX_PLANE_SETUP(N, T);
+  PHX_PLANE_LDS(N, T);
   if (fwd)
-    group_forward_plane<N, T>(d, b, m, pl, red_lds_);
+    group_forward_plane<N, T>(d, b, m, lds_, red_lds_);
   else
-    gradient_sweep_plane<N, T, WANT_OBJ, WANT_P0, FF>(d, b, m, want_probe, want_pos, pl, red_lds_);
+    gradient_sweep_plane<N, T, WANT_OBJ, WANT_P0, FF>(d, b, m, want_probe, want_pos, lds_, red_lds_);
 }
 
 // Object tiles of the gathers: one row of UPD_TW pixels (a multiple of 64, so each wave
