@@ -2124,27 +2124,32 @@ int phx_gradient_group_sweeps(const phx_group_desc* d, int poisson, int want_obj
   // ran in earlier launches.  For P = 384 planes on 256 CUs that is 3 full launches
   // where the two sweeps take 4 rounds (2 of them half full).
   const int P = d->bmax * d->n_modes, M = d->n_modes, C = device_cu_count();
-  int launches = 0;
-  if (d->n == 128 && C > 0 && P > C) {
-    for (int fwd = 0, adj = 0; adj < P; ++launches) {
+  struct Tasks {
+    int fwd, nf, adj, na;  // forward planes [fwd, fwd + nf), adjoint planes [adj, adj + na)
+  };
+  constexpr int MAX_LAUNCHES = 8;
+  Tasks plan[MAX_LAUNCHES];
+  int launches = 0, adj = 0;
+  if (d->n == 128 && C > 0 && P > C && P <= 2 * C) {
+    for (int fwd = 0; adj < P && launches < MAX_LAUNCHES; ++launches) {
       const int nf = std::min(C, P - fwd);
-      const int na = std::min(C - nf, (fwd / M) * M - adj);
+      const int na = std::max(0, std::min(C - nf, (fwd / M) * M - adj));
+      plan[launches] = {fwd, nf, adj, na};
       fwd += nf;
-      adj += std::max(na, 0);
+      adj += na;
     }
   }
-  if (launches == 0 || launches >= 2 * ((P + C - 1) / C)) {
+  if (adj < P || launches >= 2 * ((P + C - 1) / C)) {
     r = phx_group_forward(d, stream);
     if (r) return r;
     return phx_gradient_sweep_farfield(d, poisson, want_object, want_probe, want_positions, stream);
   }
   const bool p0 = want_probe || want_positions;
-  for (int fwd = 0, adj = 0; adj < P;) {
-    const int nf = std::min(C, P - fwd);
-    const int na = std::max(0, std::min(C - nf, (fwd / M) * M - adj));
+  for (int k = 0; k < launches; ++k) {
+    const Tasks& t = plan[k];
 #define K(WO, WP, FF)                                                                                         \
-  hipLaunchKernelGGL((gradient_tasks_k<128, 1024, WO, WP, FF>), dim3((unsigned)(nf + na)), dim3(1024), 0, stream, \
-                     *d, want_probe, want_positions, fwd, nf, adj)
+  hipLaunchKernelGGL((gradient_tasks_k<128, 1024, WO, WP, FF>), dim3((unsigned)(t.nf + t.na)), dim3(1024), 0, \
+                     stream, *d, want_probe, want_positions, t.fwd, t.nf, t.adj)
 #define KF(WO, WP)                     \
   do {                                 \
     if (poisson)                       \
@@ -2163,8 +2168,6 @@ int phx_gradient_group_sweeps(const phx_group_desc* d, int poisson, int want_obj
 #undef KF
 #undef K
     PHX_CHECK_LAUNCH();
-    fwd += nf;
-    adj += na;
   }
   return PHX_OK;
 }
