@@ -323,6 +323,13 @@ int phx_gradient_sweep_farfield(const phx_group_desc* d, int poisson, int want_o
  * the 4 rounds of the two sweeps).  Otherwise it launches the two sweeps. */
 int phx_gradient_group_sweeps(const phx_group_desc* d, int poisson, int want_object, int want_probe,
                               int want_positions, hipStream_t stream);
+/* The plane-task schedule phx_gradient_group_sweeps runs (host only, no device call; no
+ * counterpart in the reference): for `planes` = bmax * n_modes planes on `cus` CUs,
+ * writes 4 ints per launch to plan (first forward plane, forward count, first adjoint
+ * plane, adjoint count) and returns the number of launches, or 0 when the two sweeps
+ * as separate launches take no more rounds (planes <= cus or planes > 2 cus, or no
+ * schedule within max_launches).  PHX_EINVAL for planes not a multiple of modes. */
+int phx_plane_task_plan(int planes, int modes, int cus, int* plan, int max_launches);
 /* grad_obj[i] = sum_{b,m} scatter(g_i conj(psi_i)) for slices first .. first+count-1
  * from the wave gradients phx_gradient_sweep stored, gathered per object tile in group
  * and mode order (no atomics; deterministic).  Writes every pixel of those slices (zero

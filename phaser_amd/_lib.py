@@ -75,6 +75,7 @@ SIGNATURES = {
     "phx_dry_run_stats_intensity": (c_int, [_P, c_vp]),
     "phx_gradient_sweep_farfield": (c_int, [_P, c_int, c_int, c_int, c_int, c_vp]),
     "phx_gradient_group_sweeps": (c_int, [_P, c_int, c_int, c_int, c_int, c_vp]),
+    "phx_plane_task_plan": (c_int, [c_int, c_int, c_int, c_vp, c_int]),
     "phx_gradient_object_gather": (c_int, [_P, c_int, c_int, c_vp]),
     "phx_gradient_object_gather_begin": (c_int, [_P, c_int, c_int, _P, c_vp]),
     "phx_gemm_bf16": (c_int, [c_vp, c_i64, c_int, c_vp, c_i64, c_int, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp,

@@ -2109,6 +2109,26 @@ int phx_gradient_sweep_farfield(const phx_group_desc* d, int poisson, int want_o
   return PHX_OK;
 }
 
+int phx_plane_task_plan(int planes, int modes, int cus, int* plan, int max_launches) {
+  const int P = planes, M = modes, C = cus;
+  if (P < 1 || M < 1 || P % M != 0 || !plan || max_launches < 1) return PHX_EINVAL;
+  if (C < 1 || P <= C || P > 2 * C) return 0;  // one round per sweep, or no round to save
+  int launches = 0, fwd = 0, adj = 0;
+  for (; adj < P && launches < max_launches; ++launches) {
+    const int nf = std::min(C, P - fwd);
+    const int na = std::max(0, std::min(C - nf, (fwd / M) * M - adj));
+    int* t = plan + 4 * launches;
+    t[0] = fwd;
+    t[1] = nf;
+    t[2] = adj;
+    t[3] = na;
+    fwd += nf;
+    adj += na;
+  }
+  if (adj < P || launches >= 2 * ((P + C - 1) / C)) return 0;
+  return launches;
+}
+
 int phx_gradient_group_sweeps(const phx_group_desc* d, int poisson, int want_object, int want_probe,
                               int want_positions, hipStream_t stream) {
   int r = check_desc(d);
@@ -2123,33 +2143,22 @@ int phx_gradient_group_sweeps(const phx_group_desc* d, int poisson, int want_obj
   // remaining forward planes first, then adjoint planes whose slots' forward sweeps all
   // ran in earlier launches.  For P = 384 planes on 256 CUs that is 3 full launches
   // where the two sweeps take 4 rounds (2 of them half full).
-  const int P = d->bmax * d->n_modes, M = d->n_modes, C = device_cu_count();
-  struct Tasks {
-    int fwd, nf, adj, na;  // forward planes [fwd, fwd + nf), adjoint planes [adj, adj + na)
-  };
   constexpr int MAX_LAUNCHES = 8;
-  Tasks plan[MAX_LAUNCHES];
-  int launches = 0, adj = 0;
-  if (d->n == 128 && C > 0 && P > C && P <= 2 * C) {
-    for (int fwd = 0; adj < P && launches < MAX_LAUNCHES; ++launches) {
-      const int nf = std::min(C, P - fwd);
-      const int na = std::max(0, std::min(C - nf, (fwd / M) * M - adj));
-      plan[launches] = {fwd, nf, adj, na};
-      fwd += nf;
-      adj += na;
-    }
-  }
-  if (adj < P || launches >= 2 * ((P + C - 1) / C)) {
+  int plan[4 * MAX_LAUNCHES];
+  const int launches = d->n == 128 ? phx_plane_task_plan(d->bmax * d->n_modes, d->n_modes, device_cu_count(), plan,
+                                                         MAX_LAUNCHES)
+                                   : 0;
+  if (launches <= 0) {
     r = phx_group_forward(d, stream);
     if (r) return r;
     return phx_gradient_sweep_farfield(d, poisson, want_object, want_probe, want_positions, stream);
   }
   const bool p0 = want_probe || want_positions;
   for (int k = 0; k < launches; ++k) {
-    const Tasks& t = plan[k];
+    const int* t = plan + 4 * k;  // forward planes [t0, t0 + t1), adjoint planes [t2, t2 + t3)
 #define K(WO, WP, FF)                                                                                         \
-  hipLaunchKernelGGL((gradient_tasks_k<128, 1024, WO, WP, FF>), dim3((unsigned)(t.nf + t.na)), dim3(1024), 0, \
-                     stream, *d, want_probe, want_positions, t.fwd, t.nf, t.adj)
+  hipLaunchKernelGGL((gradient_tasks_k<128, 1024, WO, WP, FF>), dim3((unsigned)(t[1] + t[3])), dim3(1024), 0, \
+                     stream, *d, want_probe, want_positions, t[0], t[1], t[2])
 #define KF(WO, WP)                     \
   do {                                 \
     if (poisson)                       \
