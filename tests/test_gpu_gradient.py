@@ -141,29 +141,33 @@ def test_farfield_step_in_sweep_equals_separate_launch(noise):
     assert abs(res[0][3] - res[1][3]) <= 1e-9 * abs(res[0][3])
 
 
-@pytest.mark.parametrize('noise', ['amplitude', 'poisson'])
-def test_group_sweeps_as_plane_tasks_equal_two_sweeps(noise):
-    """phx_gradient_group_sweeps on 64 positions x 6 modes = 384 planes of 128^2 (more
-    than the MI355X's 256 CUs, so the forward and adjoint sweeps run as plane tasks in 3
-    launches), then a 40-position group in the same slots (its inactive slots skipped),
-    against phx_group_forward + phx_gradient_sweep_farfield.  Every plane runs the same
-    code either way, so the stored waves, the intensity terms and the object and probe
-    gradients agree to the last bit; the loss and the position gradients are FP64 sums
-    with atomics."""
+@pytest.mark.parametrize('noise,modes,sizes', [('amplitude', 6, (64, 40)), ('poisson', 6, (64, 40)),
+                                               ('amplitude', 7, (40, 24))])
+def test_group_sweeps_as_plane_tasks_equal_two_sweeps(noise, modes, sizes):
+    """phx_gradient_group_sweeps on more 128^2 planes than the MI355X's 256 CUs, as plane
+    tasks: 64 positions x 6 modes = 384 planes in 3 full launches; 40 x 7 = 280 planes in
+    launches of 256, 256 and 48 tasks.  A second, smaller group follows in the same slots
+    (its inactive slots skipped).  Against phx_group_forward + phx_gradient_sweep_farfield:
+    every plane runs the same code either way, so the stored waves, the intensity terms
+    and the object and probe gradients agree to the last bit; the loss and the position
+    gradients are FP64 sums with atomics."""
     import torch
     from phaser_amd.engines.gradient import make_model
     from phaser_amd.plan import NOISE_MODELS
-    _cfg, state, pats = _problem(S=2, M=6, n=128, scan=(8, 13), noise=noise, seed=29)
+    _cfg, state, pats = _problem(S=2, M=modes, n=128, scan=(8, 13), noise=noise, seed=29)
     idx = numpy.arange(8 * 13)
-    groups = [numpy.array(numpy.unravel_index(idx[:64], (8, 13))), numpy.array(numpy.unravel_index(idx[64:], (8, 13)))]
-    print(f"{torch.cuda.get_device_properties(0).multi_processor_count} CUs for 384 planes", flush=True)
+    b0, b1 = sizes
+    groups = [numpy.array(numpy.unravel_index(idx[:b0], (8, 13))),
+              numpy.array(numpy.unravel_index(idx[b0:b0 + b1], (8, 13)))]
+    planes = b0 * modes
+    print(f"{torch.cuda.get_device_properties(0).multi_processor_count} CUs for {planes} planes", flush=True)
     res = []
     for tasks in (False, True):
         model, _ = make_model(state, pats.patterns, pats.pattern_mask, NOISE_MODELS[noise](), groups=groups)
-        assert model.fused and model.ws.B * model.ws.M == 384
+        assert model.fused and model.ws.B * model.ws.M == planes
         model.group_sweeps = tasks
         out = []
-        for g, gs in ((0, 64), (1, 40)):
+        for g, gs in ((0, b0), (1, b1)):
             model.forward(g, loss_in_backward=True)
             model.backward(g)
             ws = model.ws
