@@ -436,8 +436,8 @@ def main():
         learned_at_scale(args.learned_scale, args.learned_epochs)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
-              "build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engines run on one GPU and "
+              "have no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     if args.workload == "128x6-s16-grad":
         gradient_engine_workload(args)

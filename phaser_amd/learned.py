@@ -10,7 +10,7 @@ the patch targets and the stitching.  torch only allocates.
 Network: input log(1 + 2x2-binned pattern) (n^2/4 values; 4096 for 128^2 patterns),
 dense layers with ReLU between them, linear output [phase (ps^2), amplitude (ps^2)] of
 the projected object prod_s O_s over the ps x ps window centred on the beam.  Loss: mean
-squared error.  The plain reconstruction stays the judge: `nrmse` measures any start
+squared error.  The reconstruction engines decide the result: `nrmse` measures any start
 or result against a reference object, and a caller falls back to the flat start when
 the network does not improve on it.
 """
