@@ -141,20 +141,23 @@ def test_farfield_step_in_sweep_equals_separate_launch(noise):
     assert abs(res[0][3] - res[1][3]) <= 1e-9 * abs(res[0][3])
 
 
-@pytest.mark.parametrize('noise,modes,sizes', [('amplitude', 6, (64, 40)), ('poisson', 6, (64, 40)),
-                                               ('amplitude', 7, (40, 24))])
-def test_group_sweeps_as_plane_tasks_equal_two_sweeps(noise, modes, sizes):
+@pytest.mark.parametrize('noise,modes,sizes,tilt', [('amplitude', 6, (64, 40), False), ('poisson', 6, (64, 40), False),
+                                                    ('amplitude', 7, (40, 24), False), ('amplitude', 6, (64, 40), True)])
+def test_group_sweeps_as_plane_tasks_equal_two_sweeps(noise, modes, sizes, tilt):
     """phx_gradient_group_sweeps on more 128^2 planes than the MI355X's 256 CUs, as plane
     tasks: 64 positions x 6 modes = 384 planes in 3 full launches; 40 x 7 = 280 planes in
     launches of 256, 256 and 48 tasks.  A second, smaller group follows in the same slots
     (its inactive slots skipped).  Against phx_group_forward + phx_gradient_sweep_farfield:
     every plane runs the same code either way, so the stored waves, the intensity terms
     and the object and probe gradients agree to the last bit; the loss and the position
-    gradients are FP64 sums with atomics."""
+    gradients are FP64 sums with atomics.  With per-position tilts both read the group's
+    tilted propagators."""
     import torch
     from phaser_amd.engines.gradient import make_model
     from phaser_amd.plan import NOISE_MODELS
-    _cfg, state, pats = _problem(S=2, M=modes, n=128, scan=(8, 13), noise=noise, seed=29)
+    _cfg, state, pats = _problem(S=3 if tilt else 2, M=modes, n=128, scan=(8, 13), noise=noise, seed=29)
+    if tilt:
+        state.tilt = numpy.random.default_rng(31).uniform(-4.0, 4.0, size=state.scan.shape).astype(numpy.float32)
     idx = numpy.arange(8 * 13)
     b0, b1 = sizes
     groups = [numpy.array(numpy.unravel_index(idx[:b0], (8, 13))),
